@@ -1,0 +1,587 @@
+"""numpy/scipy restatement of PARLA's randomized-NLA hot path (the oracle).
+
+TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, by ``__graft_entry__.smoke``
+and by the CPU-baseline leg of ``bench.py``; never by the GPU product path.
+
+Every function cites the reference file:line it restates (paths relative to
+the PARLA source tree).  Random draws are made through numpy's public
+``Generator`` API in the same order as the reference, so sketch indices, signs
+and Gaussian values are bit-identical to what the reference draws for the same
+seed (numpy 2.2.6; numpy does not promise stream stability across versions).
+
+Parity status of each function (how it is pinned, see tests/test_oracle_*.py):
+
+* gaussian_operator, sjlt_operator, sjlt_draw -- golden vectors written by the
+  reference itself (tests/golden/make_golden.py) + distributional invariants.
+* orthonormal_operator -- Q^T Q = I, diag(R) > 0 makes Q unique.
+* lsqr, sym_ortho -- the three known-answer cases of the reference docstring
+  (lsqr.py:259-305), golden trajectories written by the reference's lsqr,
+  scipy.sparse.linalg.lsqr on unpreconditioned problems, closed-form LS.
+* spo -- closed-form solutions of the reference's own test problems,
+  presolve exactness on consistent problems, convergence-rate fits.
+* rs1/rf1/qb1/svd1/evd1 -- exact low-rank recovery, orthonormality,
+  B = Q^T A, power-method convergence (the reference's test families).
+* sampling_operator -- brute force (index gather == dense selection matrix).
+"""
+import time
+import warnings
+
+import numpy as np
+import scipy.linalg as la
+import scipy.sparse as sp
+
+EPS = np.finfo(np.float64).eps
+
+__all__ = [
+    "orthonormal_operator", "gaussian_operator", "sjlt_operator", "sjlt_draw",
+    "sampling_operator", "dim_checks", "sym_ortho", "lsqr", "DenseOp",
+    "a_lift_precond", "pcss2", "SketchAndPrecondLog", "spo", "orth",
+    "lu_stabilize", "rs1", "rf1", "qb1", "svd1", "evd1", "loglinear_fit",
+    "loglog_fit", "backward_error", "EPS",
+]
+
+
+# --------------------------------------------------------------------------
+# Sketching operators  (parla/utils/sketching.py)
+# --------------------------------------------------------------------------
+
+def gaussian_operator(n_rows, n_cols, rng, normalize=True):
+    """utils/sketching.py:20-31.  N(0, 1/min(r, c)) entries, or N(0, 1)."""
+    rng = np.random.default_rng(rng)
+    if not normalize:
+        return rng.standard_normal((n_rows, n_cols))
+    sigma = np.sqrt(1.0 / min(n_rows, n_cols))
+    return rng.normal(0.0, sigma, (n_rows, n_cols))
+
+
+def orthonormal_operator(n_rows, n_cols, rng):
+    """utils/sketching.py:9-17.  Haar factor: QR of a Gaussian, signs of diag(R)."""
+    if n_rows < n_cols:
+        return orthonormal_operator(n_cols, n_rows, rng).T
+    rng = np.random.default_rng(rng)
+    G = gaussian_operator(n_rows, n_cols, rng)
+    Q, R = la.qr(G, overwrite_a=True, pivoting=False, mode="economic")
+    return Q * np.sign(np.diag(R))
+
+
+def sjlt_draw(short, long, rng, vec_nnz=8):
+    """Raw draws of a wide SJLT (``short`` x ``long``), utils/sketching.py:51-71.
+
+    Returns ``rows`` (long, k) int64 -- the row indices of column j -- and
+    ``vals`` (long, k) float64 = +-1/sqrt(k).  The draw order is the
+    reference's: one ``choice`` per column (without replacement unless the
+    column is shorter than k, :56-65), then one ``random`` block for the
+    signs, ``<= 0.5`` giving -1 (:69-70).
+    """
+    rng = np.random.default_rng(rng)
+    k = min(long, vec_nnz)
+    with_repl = short < k
+    if with_repl:
+        warnings.warn(f"Can't set {k} nonzeros per column for columns of length "
+                      f"{short}. Sampling indices with replacement instead.")
+    rows = np.empty((long, k), dtype=np.int64)
+    for j in range(long):
+        rows[j] = rng.choice(short, k, replace=with_repl)
+    vals = np.ones(long * k)
+    vals[rng.random(long * k) <= 0.5] = -1.0
+    vals /= np.sqrt(k)
+    return rows, vals.reshape(long, k)
+
+
+def sjlt_operator(n_rows, n_cols, rng, vec_nnz=8):
+    """utils/sketching.py:34-80.  Wide: CSC; tall: transpose of the wide draw, CSR."""
+    if n_cols >= n_rows:
+        rows, vals = sjlt_draw(n_rows, n_cols, rng, vec_nnz)
+        k = rows.shape[1]
+        cols = np.repeat(np.arange(n_cols), k)
+        S = sp.coo_matrix((vals.ravel(), (rows.ravel(), cols)),
+                          shape=(n_rows, n_cols))
+        return S.tocsc()
+    return sjlt_operator(n_cols, n_rows, rng, vec_nnz).T.tocsr()
+
+
+def sampling_operator(n_rows, n_cols, rng, indices=None):
+    """utils/sketching.py:204-232, as a dense index array.
+
+    Returns ``(indices, wide)``: the sorted sample of ``max(r, c)`` of size
+    ``min(r, c)`` and whether the operator is wide.  The reference's
+    ``rmatmat`` (:222-224) and tall branch (:230) are broken; this restates
+    the intended selection ``S @ X == X[indices]`` for wide S.
+    """
+    rng = np.random.default_rng(rng)
+    if indices is None:
+        indices = rng.choice(max(n_rows, n_cols), min(n_rows, n_cols),
+                             replace=False)
+        indices.sort()
+    else:
+        assert indices.size == min(n_rows, n_cols)
+    return indices, n_cols >= n_rows
+
+
+# --------------------------------------------------------------------------
+# LSQR  (parla/comps/determiter/lsqr.py)
+# --------------------------------------------------------------------------
+
+def sym_ortho(a, b):
+    """Stable Givens rotation (c, s, r), comps/determiter/lsqr.py:63-95."""
+    if b == 0:
+        return np.sign(a), 0, abs(a)
+    if a == 0:
+        return 0, np.sign(b), abs(b)
+    if abs(b) > abs(a):
+        t = a / b
+        s = np.sign(b) / np.sqrt(1 + t * t)
+        return s * t, s, b / s
+    t = b / a
+    c = np.sign(a) / np.sqrt(1 + t * t)
+    return c, c * t, a / c
+
+
+class DenseOp:
+    """Minimal operator with matvec / rmatvec over a dense array."""
+
+    def __init__(self, A):
+        self.A = np.asarray(A, dtype=np.float64)
+        self.shape = self.A.shape
+
+    def matvec(self, x):
+        return self.A @ x
+
+    def rmatvec(self, y):
+        return self.A.T @ y
+
+
+def lsqr(A, b, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8, iter_lim=None,
+         calc_var=False, x0=None):
+    """Paige-Saunders LSQR as modified by PARLA, comps/determiter/lsqr.py:98-574.
+
+    ``A`` has ``shape``, ``matvec`` and ``rmatvec``.  Returns the 10-tuple
+    (x, istop, itn, r1norm, r2norm, anorm, acond, arnorms, xnorm, var) whose
+    8th entry is the per-iteration log of ||A^T r|| estimates, recorded at
+    the top of each iteration (:410-413, :572).  On the zero-residual early
+    exit (:392-395) that entry is the scalar arnorm, as in the reference.
+    """
+    if not hasattr(A, "matvec"):
+        A = DenseOp(A)
+    b = np.atleast_1d(np.asarray(b, dtype=np.float64))
+    if b.ndim > 1:
+        b = b.squeeze()
+    m, n = A.shape
+    if iter_lim is None:
+        iter_lim = 2 * n
+    var = np.zeros(n)
+    ctol = 1.0 / conlim if conlim > 0 else 0.0
+    dampsq = damp * damp
+    # recurrence state (lsqr.py:342-356)
+    itn, istop = 0, 0
+    anorm = acond = ddnorm = res2 = xnorm = xxnorm = z = 0.0
+    cs2, sn2 = -1.0, 0.0
+
+    bnorm = np.linalg.norm(b)
+    if x0 is None:
+        x = np.zeros(n)
+        u = b
+        beta = bnorm
+    else:
+        x = np.asarray(x0, dtype=np.float64)
+        u = b - A.matvec(x)
+        beta = np.linalg.norm(u)
+    if beta > 0:
+        u = (1 / beta) * u
+        v = A.rmatvec(u)
+        alfa = np.linalg.norm(v)
+    else:
+        v = x.copy()
+        alfa = 0.0
+    if alfa > 0:
+        v = (1 / alfa) * v
+    w = v.copy()
+    rhobar, phibar = alfa, beta
+    rnorm = r1norm = r2norm = beta
+    arnorm = alfa * beta
+    if arnorm == 0:
+        return x, istop, itn, r1norm, r2norm, anorm, acond, arnorm, xnorm, var
+
+    arnorms = -np.ones(iter_lim)
+    while itn < iter_lim:
+        arnorms[itn] = arnorm
+        itn += 1
+        # bidiagonalisation step (lsqr.py:421-430)
+        u = A.matvec(v) - alfa * u
+        beta = np.linalg.norm(u)
+        if beta > 0:
+            u = (1 / beta) * u
+            anorm = np.sqrt(anorm ** 2 + alfa ** 2 + beta ** 2 + damp ** 2)
+            v = A.rmatvec(u) - beta * v
+            alfa = np.linalg.norm(v)
+            if alfa > 0:
+                v = (1 / alfa) * v
+        # damping rotation, then the bidiagonal QR step (lsqr.py:434-448)
+        rhobar1 = np.sqrt(rhobar ** 2 + damp ** 2)
+        cs1, sn1 = rhobar / rhobar1, damp / rhobar1
+        psi = sn1 * phibar
+        phibar = cs1 * phibar
+        cs, sn, rho = sym_ortho(rhobar1, beta)
+        theta = sn * alfa
+        rhobar = -cs * alfa
+        phi = cs * phibar
+        phibar = sn * phibar
+        tau = sn * phi
+        # solution / search-direction update (lsqr.py:451-460)
+        t1, t2 = phi / rho, -theta / rho
+        dk = (1 / rho) * w
+        x = x + t1 * w
+        w = v + t2 * w
+        ddnorm = ddnorm + np.linalg.norm(dk) ** 2
+        if calc_var:
+            var = var + dk ** 2
+        # norm(x) estimate (lsqr.py:465-474)
+        delta = sn2 * rho
+        gambar = -cs2 * rho
+        rhs = phi - delta * z
+        zbar = rhs / gambar
+        xnorm = np.sqrt(xxnorm + zbar ** 2)
+        gamma = np.sqrt(gambar ** 2 + theta ** 2)
+        cs2, sn2 = gambar / gamma, theta / gamma
+        z = rhs / gamma
+        xxnorm = xxnorm + z ** 2
+        # convergence tests (lsqr.py:479-526)
+        acond = anorm * np.sqrt(ddnorm)
+        res1 = phibar ** 2
+        res2 = res2 + psi ** 2
+        rnorm = np.sqrt(res1 + res2)
+        arnorm = alfa * abs(tau)
+        r1sq = rnorm ** 2 - dampsq * xxnorm
+        r1norm = np.sqrt(abs(r1sq))
+        if r1sq < 0:
+            r1norm = -r1norm
+        r2norm = rnorm
+        test1 = rnorm / bnorm
+        test2 = arnorm / (anorm * rnorm + EPS)
+        test3 = 1.0 / (acond + EPS)
+        t1 = test1 / (1 + anorm * xnorm / bnorm)
+        rtol = btol + atol * anorm * xnorm / bnorm
+        if itn >= iter_lim:
+            istop = 7
+        if 1 + test3 <= 1:
+            istop = 6
+        if 1 + test2 <= 1:
+            istop = 5
+        if 1 + t1 <= 1:
+            istop = 4
+        if test3 <= ctol:
+            istop = 3
+        if test2 <= atol:
+            istop = 2
+        if test1 <= rtol:
+            istop = 1
+        if istop != 0:
+            break
+    arnorms = arnorms[arnorms > -1]
+    return x, istop, itn, r1norm, r2norm, anorm, acond, arnorms, xnorm, var
+
+
+# --------------------------------------------------------------------------
+# Preconditioned operator and PcSS2  (comps/preconditioning.py, saddle.py)
+# --------------------------------------------------------------------------
+
+class _PrecondOp:
+    """A_pc = [A; sqrt(delta) I] M with M = R^{-1} (upper_tri) or M = R."""
+
+    def __init__(self, A, delta, R, upper_tri):
+        self.A = A
+        self.R = R
+        self.m = A.shape[0]
+        self.sd = np.sqrt(delta)
+        self.upper_tri = upper_tri
+        lift = A.shape[0] + (A.shape[1] if delta > 0 else 0)
+        self.shape = (lift, R.shape[1])
+
+    def m_fwd(self, z):
+        if self.upper_tri:
+            return la.solve_triangular(self.R, z, lower=False)
+        return self.R @ z
+
+    def m_adj(self, w):
+        if self.upper_tri:
+            return la.solve_triangular(self.R, w, "T", lower=False)
+        return self.R.T @ w
+
+    def matvec(self, arg):
+        """preconditioning.py:26-31 (tri) / :45-48 (general)."""
+        work = self.m_fwd(arg)
+        out = self.A @ work
+        if self.sd > 0:
+            out = np.concatenate((out, self.sd * work))
+        return out
+
+    def rmatvec(self, arg):
+        """preconditioning.py:33-38 (tri) / :50-54 (general)."""
+        work = self.A.T @ arg[:self.m]
+        if self.sd > 0:
+            work = work + self.sd * arg[self.m:]
+        return self.m_adj(work)
+
+
+def a_lift_precond(A, delta, R, upper_tri=False, k=1):
+    """comps/preconditioning.py:16-67 -> (A_pc, M_fwd, M_adj)."""
+    if k != 1:
+        raise NotImplementedError()
+    op = _PrecondOp(A, delta, R, upper_tri)
+    return op, op.m_fwd, op.m_adj
+
+
+def pcss2(A, b, delta, tol, iter_lim, R, upper_tri, z0):
+    """Overdetermined branch of PcSS2.__call__, comps/determiter/saddle.py:187-201."""
+    m, n = A.shape
+    A_pc, M_fwd, _ = a_lift_precond(A, delta, R, upper_tri)
+    if delta > 0:
+        b = np.concatenate((b, np.zeros(n)))
+    res = lsqr(A_pc, b, atol=tol, btol=tol, iter_lim=iter_lim, x0=z0)
+    x = M_fwd(res[0])
+    y = b[:m] - A @ x
+    return x, y, res[7], res
+
+
+# --------------------------------------------------------------------------
+# SPO driver and its log  (drivers/least_squares.py, determiter/logging.py)
+# --------------------------------------------------------------------------
+
+class SketchAndPrecondLog:
+    """comps/determiter/logging.py:4-94 (times and the error log)."""
+
+    def __init__(self):
+        self.time_sketch = 0.0
+        self.time_factor = 0.0
+        self.time_presolve = 0.0
+        self.time_convert = 0.0
+        self.time_iterate = 0.0
+        self.times = None
+        self.errors = None
+
+    @property
+    def time_setup(self):
+        return self.time_sketch + self.time_factor + self.time_convert
+
+    def wrap_up(self, iter_errors, init_error):
+        """logging.py:72-94: amortise iterate time linearly; prepend error at 0."""
+        iter_errors = np.atleast_1d(iter_errors)
+        setup = self.time_setup
+        ramp = np.linspace(0, self.time_iterate, iter_errors.size, endpoint=True)
+        self.times = np.concatenate(([setup], setup + self.time_presolve + ramp))
+        self.errors = np.concatenate(([init_error], iter_errors))
+
+
+def dim_checks(sampling_factor, n_rows, n_cols):
+    """drivers/least_squares.py:89-104."""
+    assert n_rows >= n_cols
+    d = int(sampling_factor * n_cols)
+    if d > n_rows:
+        warnings.warn(f"The embedding dimension d={d} exceeds the {n_rows} rows "
+                      f"of the data matrix; proceeding with d={n_rows}.")
+        d = n_rows
+    assert d >= n_cols
+    return d
+
+
+def _sketch_gen(kind, vec_nnz=8, normalize=True):
+    if callable(kind):
+        return kind
+    if kind == "gaussian":
+        return lambda r, c, rng: gaussian_operator(r, c, rng, normalize)
+    if kind == "sjlt":
+        return lambda r, c, rng: sjlt_operator(r, c, rng, vec_nnz)
+    raise ValueError(kind)
+
+
+def spo(A, b, delta, tol, iter_lim, rng, sketch="gaussian", sampling_factor=4,
+        mode="qr", logging=True, vec_nnz=8, return_info=False):
+    """SPO.__call__, drivers/least_squares.py:291-369 (modes 'qr' and 'chol').
+
+    Returns ``(x, log)``; with ``return_info`` also a dict holding the
+    presolve decision, R, z_ske and the raw LSQR tuple.
+    """
+    m, n = A.shape
+    sqrt_delta = np.sqrt(delta)
+    d = dim_checks(sampling_factor, m, n)
+    rng = np.random.default_rng(rng)
+    clock = time.time if logging else (lambda: 0.0)
+    log = SketchAndPrecondLog()
+    gen = _sketch_gen(sketch, vec_nnz)
+
+    tic = clock()
+    S = gen(d, m, rng)
+    A_ske = S @ A
+    log.time_sketch = clock() - tic
+
+    if mode == "qr":
+        tic = clock()
+        if delta > 0:
+            A_ske = np.vstack((A_ske, sqrt_delta * np.eye(n)))
+        Q, R = la.qr(A_ske, overwrite_a=True, mode="economic")
+        log.time_factor = clock() - tic
+        tic = clock()
+        b_ske = S @ b
+        z_ske = Q[:d, :].T @ b_ske
+        x_ske = la.solve_triangular(R, z_ske, lower=False)
+    elif mode == "chol":
+        tic = clock()
+        G = A_ske.T @ A_ske + delta * np.eye(n)
+        R = la.cholesky(G, lower=False)
+        log.time_factor = clock() - tic
+        tic = clock()
+        b_ske = S @ b
+        z_ske = la.solve_triangular(R, A_ske.T @ b_ske, lower=False, trans="T")
+        x_ske = la.solve_triangular(R, z_ske, lower=False)
+    else:
+        raise ValueError()
+
+    # presolve decision (least_squares.py:344-351)
+    r = A @ x_ske - b
+    if delta > 0:
+        r = np.concatenate((r, sqrt_delta * x_ske))
+    rel_err = la.norm(r) / la.norm(b)
+    used_presolve = not (rel_err >= 1 or (rel_err > 1e-15 and R.shape[0] != R.shape[1]))
+    if not used_presolve:
+        z_ske = None
+    log.time_presolve = clock() - tic
+
+    tic = clock()
+    x, y, arnorms, raw = pcss2(A, b, delta, tol, iter_lim, R, True, z_ske)
+    log.time_iterate = clock() - tic
+
+    if logging:
+        ar0 = la.solve_triangular(R, A.T @ b, "T", lower=False)
+        log.wrap_up(arnorms, la.norm(ar0))
+    if return_info:
+        info = dict(R=R, z_ske=z_ske, x_ske=x_ske, rel_err=rel_err,
+                    used_presolve=used_presolve, lsqr=raw, d=d)
+        return x, log, info
+    return x, log
+
+
+def backward_error(A, b, x):
+    """eta(x) = ||A^T r|| / (||A||_F ||r||), r = b - A x, all in float64."""
+    A = np.asarray(A, dtype=np.float64)
+    r = b - A @ x
+    rn = la.norm(r)
+    if rn == 0:
+        return 0.0
+    return la.norm(A.T @ r) / (la.norm(A, "fro") * rn)
+
+
+# --------------------------------------------------------------------------
+# QB chain  (sketchers/aware.py, rangefinders.py, qb.py, svd.py, evd.py)
+# --------------------------------------------------------------------------
+
+def orth(S):
+    """utils/linalg_wrappers.py:6-7."""
+    return la.qr(S, mode="economic")[0]
+
+
+def lu_stabilize(S):
+    """utils/linalg_wrappers.py:10-12: P L with partial pivoting."""
+    return la.lu(S, permute_l=True)[0]
+
+
+def rs1(A, k, rng, sketch_op_gen=None, num_pass=2, stabilizer=orth,
+        passes_per_stab=1):
+    """RS1.__call__, comps/sketchers/aware.py:160-184."""
+    if sketch_op_gen is None:
+        sketch_op_gen = _sketch_gen("gaussian")
+    assert num_pass >= 0
+    rng = np.random.default_rng(rng)
+    done = 0
+    if num_pass % 2 == 0:
+        S = sketch_op_gen(A.shape[1], k, rng)
+    else:
+        S = A.T @ sketch_op_gen(A.shape[0], k, rng)
+        done += 1
+        if passes_per_stab == 1:
+            S = stabilizer(S)
+    q = (num_pass - done) // 2
+    while q > 0:
+        S = A @ S
+        done += 1
+        if done % passes_per_stab == 0:
+            S = stabilizer(S)
+        S = A.T @ S
+        done += 1
+        if done % passes_per_stab == 0:
+            S = stabilizer(S)
+        q -= 1
+    return S
+
+
+def rf1(A, k, tol, rng, rso):
+    """RF1.__call__, comps/rangefinders.py:176-188.  ``rso(A, k, rng)``."""
+    assert 0 < k <= min(A.shape)
+    if not np.isnan(tol):
+        warnings.warn("This RangeFinder implementation cannot directly control "
+                      "approximation error. Parameter \"tol\" is being ignored.")
+    rng = np.random.default_rng(rng)
+    S = rso(A, k, rng)
+    Y = A @ S
+    return la.qr(Y, mode="economic")[0]
+
+
+def qb1(A, k, tol, rng, rf):
+    """QB1.__call__, comps/qb.py:344-352.  ``rf(A, k, tol, rng)``."""
+    assert 0 < k <= min(A.shape)
+    if not np.isnan(tol):
+        assert 0 <= tol < 1
+    rng = np.random.default_rng(rng)
+    Q = rf(A, k, tol, rng)
+    return Q, Q.T @ A
+
+
+def svd1(A, k, tol, over, rng, qb):
+    """SVD1.__call__, drivers/svd.py:161-176.  ``qb(A, k, tol, rng)``."""
+    rng = np.random.default_rng(rng)
+    Q, B = qb(A, k + over, tol, rng)
+    U, s, Vh = la.svd(B, full_matrices=False)
+    if over > 0:
+        cut = min(k, s.size)
+        U, s, Vh = U[:, :cut], s[:cut], Vh[:cut, :]
+    keep = ~(s < 10 * EPS)
+    if not np.all(keep):
+        U, s, Vh = U[:, keep], s[keep], Vh[keep, :]
+    return Q @ U, s, Vh
+
+
+def evd1(A, k, tol, over, rng, qb):
+    """EVD1.__call__, drivers/evd.py:270-289.  ``qb(A, k, tol, rng)``."""
+    assert 0 < k <= min(A.shape)
+    if not np.isnan(tol):
+        assert 0 <= tol < np.inf
+    rng = np.random.default_rng(rng)
+    Q, B = qb(A, k + over, tol / 2, rng)
+    C = B @ Q
+    lamb, U = la.eigh(C)
+    alamb = np.abs(lamb)
+    r = min(k, Q.shape[1], np.count_nonzero(alamb > 10 * EPS))
+    order = np.argsort(-alamb)[:r]
+    return Q @ U[:, order], lamb[order]
+
+
+# --------------------------------------------------------------------------
+# Fits used by the convergence tests  (parla/utils/stats.py)
+# --------------------------------------------------------------------------
+
+def loglinear_fit(x, y):
+    """utils/stats.py:6-28: fit log(y) = a + b x; return ([a, b], R^2)."""
+    x = np.asarray(x, dtype=float).ravel()
+    y = np.asarray(y, dtype=float).ravel()
+    keep = y > 0
+    x, y = x[keep], y[keep]
+    ly = np.log(y)
+    X = np.column_stack([np.ones(x.size), x])
+    fit = la.lstsq(X, ly)[0]
+    ss_tot = np.sum((ly - ly.mean()) ** 2)
+    ss_res = np.sum((ly - X @ fit) ** 2)
+    return fit, 1 - ss_res / ss_tot
+
+
+def loglog_fit(x, y):
+    """utils/stats.py:31-56: fit log(y) = a + b log(x)."""
+    return loglinear_fit(np.log(np.asarray(x, dtype=float)), y)
