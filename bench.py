@@ -1,0 +1,211 @@
+#!/usr/bin/env python
+"""Benchmark of the randomized-NLA hot path on one MI355X.
+
+Headline (``value``): QB GB/s on the QB -> SVD workload -- A 8192 x 4096 FP64
+with s_i = i^-2 (i <= 512, then 1e-6) plus 1e-8 Gaussian noise, target rank
+256 + 10 oversampling, 2 power iterations with QR re-orthogonalisation
+(RS1 num_pass = 4), SJLT start with 8 nonzeros +-1/sqrt(8) per column.  One
+step = one QB1 call (RS1 -> RF1 -> B = Q^T A) on A resident in HBM; it reads A
+6 times, so QB GB/s = 6 * m * n * 8 B / t_step.
+
+Also reported (``extra``): LS solves/s of plain FP64 sketch-and-precondition
+(A 20000 x 200, Gaussian d = 800, Householder QR, LSQR to tol 1e-12) with its
+backward error, the SVD1 time, and for each the CPU oracle timed on the host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LS solves/s at ≤1e-10 backward err (chosen vs plain FP64 GPU); QB GB/s"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        n = [d.get("num_threads", 1) for d in threadpool_info() if d.get("user_api") == "blas"]
+        if n:
+            return int(max(n))
+    except Exception:
+        pass
+    return len(os.sched_getaffinity(0))
+
+
+def timed_steps(fn, steps, warmup, torch):
+    for i in range(warmup):
+        fn(1000 + i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        fn(i)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle timings")
+    ap.add_argument("--ls-steps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no "
+              "multi-GPU path (see DESIGN.md for how the workload would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import parla_amd as pa
+    from parla_amd import matmakers as mm
+    from parla_amd import ops
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    ops.lib()
+
+    # ------------------------------------------------------------------ QB
+    m, n, k, over = 8192, 4096, 256, 10
+    log("generating the QB workload on the host (Haar factors by numpy QR)")
+    A_host, s_true = mm.config_qb(m, n)
+    A = torch.from_numpy(A_host).to(dev)
+    qb = pa.QB1(pa.RF1(pa.RS1(pa.SkOpSJ(8, backend="device"), 4, pa.orth, 1)))
+    out = {}
+
+    def qb_step(seed):
+        out["QB"] = qb(A, k + over, np.nan, seed)
+
+    log(f"QB: {args.warmup} warmup + {args.steps} timed steps")
+    t_qb = timed_steps(qb_step, args.steps, args.warmup, torch)
+    qb_bytes = 6.0 * m * n * 8
+    value = qb_bytes / t_qb / 1e9
+
+    Q, B = out["QB"]
+    Qh, Bh = Q.cpu().numpy(), B.cpu().numpy()
+    fro = np.linalg.norm(A_host)
+    qb_err = float(np.linalg.norm(A_host - Qh @ Bh) / fro)
+    orth_err = float(np.linalg.norm(Qh.T @ Qh - np.eye(Qh.shape[1])))
+    tail = float(np.sqrt(np.sum(s_true[k + over:] ** 2)) / np.sqrt(np.sum(s_true ** 2)))
+
+    # roofline of the dominant kernel (the FP64 MFMA GEMM): per-launch events
+    events = []
+    ops.set_event_log(events)
+    for i in range(3):
+        qb_step(500 + i)
+    torch.cuda.synchronize()
+    ops.set_event_log(None)
+    g = [e for e in events if e[0] == "gemm_f64" and e[1] > 1e9]
+    g_ms = [e[3].elapsed_time(e[4]) for e in g]
+    avg_ms = float(np.mean(g_ms))
+    avg_flops = float(np.mean([e[1] for e in g]))
+    achieved = avg_flops / (avg_ms / 1e3) / 1e12
+
+    # SVD1 on the same workload
+    svd = pa.SVD1(qb)
+
+    def svd_step(seed):
+        out["SVD"] = svd(A, k, np.nan, over, seed)
+
+    t_svd = timed_steps(svd_step, max(3, args.steps // 2), 1, torch)
+    s_est = out["SVD"][1].cpu().numpy()
+    sv_rel = float(np.max(np.abs(s_est[:k] - s_true[:k]) / s_true[:k]))
+
+    # ------------------------------------------------------------------ LS
+    log("LS: plain FP64 SPO on A 20000 x 200")
+    Al_h, bl_h, _ = mm.config_ls_plain()
+    Al = torch.from_numpy(Al_h).to(dev)
+    bl = torch.from_numpy(bl_h).to(dev)
+    spo = pa.SPO(pa.SkOpGA(backend="device"), 4)
+
+    def ls_step(seed):
+        out["LS"] = spo(Al, bl, 0.0, 1e-12, 100, seed, logging=False)
+
+    t_ls = timed_steps(ls_step, args.ls_steps, 2, torch)
+    x_ls = out["LS"][0].cpu().numpy()
+    r = bl_h - Al_h @ x_ls
+    eta = float(np.linalg.norm(Al_h.T @ r) / (np.linalg.norm(Al_h) * np.linalg.norm(r)))
+    _, lg = spo(Al, bl, 0.0, 1e-12, 100, 7, logging=True)
+    ls_iters = int(lg.errors.size - 1)
+
+    # ------------------------------------------------------------------ CPU
+    cpu = None
+    extra_cpu = {}
+    if not args.no_cpu:
+        import oracle as orc
+        threads = cpu_threads()
+        log(f"CPU oracle QB1 (numpy/scipy, {threads} BLAS threads)")
+        rso = lambda M, kk, rng: orc.rs1(M, kk, rng, lambda r_, c_, g_: orc.sjlt_operator(r_, c_, g_, 8),
+                                         4, orc.orth, 1)
+        rf = lambda M, kk, tol, rng: orc.rf1(M, kk, tol, rng, rso)
+        ts = []
+        for i in range(4):
+            t0 = time.perf_counter()
+            Qo, Bo = orc.qb1(A_host, k + over, np.nan, np.random.default_rng(i), rf)
+            ts.append(time.perf_counter() - t0)
+        t_cpu = float(np.median(ts[1:]))
+        cpu = {"value": qb_bytes / t_cpu / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+               "sample": "full QB1 workload (8192 x 4096, k = 266, SJLT drawn with numpy), "
+                         "1 warm-up + median of 3"}
+        extra_cpu["qb_oracle_rel_err"] = float(np.linalg.norm(A_host - Qo @ Bo) / fro)
+        log("CPU oracle SPO on A 20000 x 200")
+        ts = []
+        for i in range(4):
+            t0 = time.perf_counter()
+            xo, _ = orc.spo(Al_h, bl_h, 0.0, 1e-12, 100, np.random.default_rng(i), sketch="gaussian",
+                            sampling_factor=4, logging=False)
+            ts.append(time.perf_counter() - t0)
+        t_cpu_ls = float(np.median(ts[1:]))
+        extra_cpu["ls_cpu_solves_per_s"] = 1.0 / t_cpu_ls
+        extra_cpu["ls_cpu_eta"] = float(orc.backward_error(Al_h, bl_h, xo))
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_qb * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Haar factors, prescribed spectrum)",
+        "config": {"workload": "QB1 (RS1 num_pass=4 -> RF1 -> B = Q^T A), A 8192x4096 f64, "
+                               "k = 256 + 10, SJLT 8 nnz drawn on the device", "m": m, "n": n,
+                   "k": k + over, "passes_over_A": 6, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "kernel": "gemm_f64_kernel (FP64 MFMA)",
+                     "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 4), "flops_per_launch": avg_flops,
+                     "launches_timed": len(g)},
+        "cpu_baseline": cpu,
+        "extra": {
+            "qb_rel_err_fro": qb_err, "qb_orth_err": orth_err, "qb_tail_bound": tail,
+            "svd1_ms": round(t_svd * 1e3, 3), "svd1_max_rel_err_top256": sv_rel,
+            "ls_plain_f64_solves_per_s": round(1.0 / t_ls, 3), "ls_plain_f64_ms": round(t_ls * 1e3, 3),
+            "ls_plain_f64_eta": eta, "ls_plain_f64_iters": ls_iters,
+            "ls_workload": "SPO qr, A 20000x200 f64, s geometric 1..1e-6, Gaussian d=800 drawn on the "
+                           "device, LSQR tol 1e-12, logging off",
+            **extra_cpu,
+        },
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,371 @@
+// Dense FP64 kernels: MFMA GEMM, GEMV, 2-norm, upper-triangular solves.
+//
+// GEMM: 64x64 output tile per 256-thread workgroup (four 64-wide wavefronts,
+// each owning a 32x32 quadrant = 2x2 tiles of v_mfma_f64_16x16x4f64), K staged
+// through LDS 16 at a time with the next K tile prefetched into registers
+// while the current one is multiplied.  Optional split-K writes per-slice
+// partials that a second kernel sums in slice order (deterministic).
+#include "common.h"
+#include "trsv_dev.h"
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+// op(A)[m][k] for the tile (m0.., k0..) -> registers (4 values per thread)
+template <int TA>
+__device__ __forceinline__ void load_a(const double* __restrict__ A, int64_t lda, int64_t M, int64_t K,
+                                       int64_t m0, int64_t k0, double (&r)[4]) {
+  const int t = threadIdx.x;
+  if (TA == 0) {  // A is M x K: rows of 16 contiguous k
+    const int64_t m = m0 + t / 4;
+    const int64_t kq = k0 + (t % 4) * 4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      r[e] = (m < M && kq + e < K) ? A[m * lda + kq + e] : 0.0;
+  } else {  // A is K x M: rows of 64 contiguous m
+    const int64_t k = k0 + t / 16;
+    const int64_t mq = m0 + (t % 16) * 4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      r[e] = (k < K && mq + e < M) ? A[k * lda + mq + e] : 0.0;
+  }
+}
+
+template <int TA>
+__device__ __forceinline__ void store_a(double (*As)[GBM + 1], const double (&r)[4]) {
+  const int t = threadIdx.x;
+  if (TA == 0) {
+    const int m = t / 4, kq = (t % 4) * 4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) As[kq + e][m] = r[e];
+  } else {
+    const int k = t / 16, mq = (t % 16) * 4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) As[k][mq + e] = r[e];
+  }
+}
+
+// op(B)[k][n] for the tile (k0.., n0..)
+template <int TB>
+__device__ __forceinline__ void load_b(const double* __restrict__ B, int64_t ldb, int64_t N, int64_t K,
+                                       int64_t n0, int64_t k0, double (&r)[4]) {
+  const int t = threadIdx.x;
+  if (TB == 0) {  // B is K x N
+    const int64_t k = k0 + t / 16;
+    const int64_t nq = n0 + (t % 16) * 4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      r[e] = (k < K && nq + e < N) ? B[k * ldb + nq + e] : 0.0;
+  } else {  // B is N x K
+    const int64_t n = n0 + t / 4;
+    const int64_t kq = k0 + (t % 4) * 4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      r[e] = (n < N && kq + e < K) ? B[n * ldb + kq + e] : 0.0;
+  }
+}
+
+template <int TB>
+__device__ __forceinline__ void store_b(double (*Bs)[GBN + 1], const double (&r)[4]) {
+  const int t = threadIdx.x;
+  if (TB == 0) {
+    const int k = t / 16, nq = (t % 16) * 4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) Bs[k][nq + e] = r[e];
+  } else {
+    const int n = t / 4, kq = (t % 4) * 4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) Bs[kq + e][n] = r[e];
+  }
+}
+
+// grid: (ceil(N/64), ceil(M/64), splits).  Slice z covers K range
+// [z*kchunk, min(K, (z+1)*kchunk)).  With splits > 1 the raw product of each
+// slice goes to part[z] (M x N, ld N); otherwise C = alpha*prod + beta*C.
+template <int TA, int TB>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk,
+                                                       double alpha, const double* __restrict__ A, int64_t lda,
+                                                       const double* __restrict__ B, int64_t ldb, double beta,
+                                                       double* __restrict__ C, int64_t ldc,
+                                                       double* __restrict__ part) {
+  __shared__ double As[GBK][GBM + 1];
+  __shared__ double Bs[GBK][GBN + 1];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t m0 = (int64_t)blockIdx.y * GBM;
+  const int64_t n0 = (int64_t)blockIdx.x * GBN;
+  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
+  const int64_t kend = min(K, kbeg + kchunk);
+
+  d4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+  double ra[4], rb[4];
+  if (kbeg < kend) {
+    load_a<TA>(A, lda, M, kend, m0, kbeg, ra);
+    load_b<TB>(B, ldb, N, kend, n0, kbeg, rb);
+  }
+  for (int64_t k0 = kbeg; k0 < kend; k0 += GBK) {
+    store_a<TA>(As, ra);
+    store_b<TB>(Bs, rb);
+    __syncthreads();
+    if (k0 + GBK < kend) {  // prefetch the next K tile while this one is used
+      load_a<TA>(A, lda, M, kend, m0, k0 + GBK, ra);
+      load_b<TB>(B, ldb, N, kend, n0, k0 + GBK, rb);
+    }
+#pragma unroll
+    for (int kk = 0; kk < GBK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      double a0 = As[kr][wm * 32 + (lane & 15)];
+      double a1 = As[kr][wm * 32 + 16 + (lane & 15)];
+      double b0 = Bs[kr][wn * 32 + (lane & 15)];
+      double b1 = Bs[kr][wn * 32 + 16 + (lane & 15)];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  // D layout of v_mfma_f64_16x16x4f64: lane l, item r -> row l/16 + 4*r, col l%16
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
+        const int64_t col = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (row < M && col < N) {
+          const double v = acc[i][j][r];
+          if (part) {
+            part[((int64_t)blockIdx.z * M + row) * N + col] = v;
+          } else {
+            double* c = C + row * ldc + col;
+            *c = (beta == 0.0) ? alpha * v : alpha * v + beta * (*c);
+          }
+        }
+      }
+}
+
+// C = alpha * sum_z part[z] + beta * C, summed in slice order.
+__global__ void splitk_reduce_kernel(int64_t M, int64_t N, int splits, double alpha, const double* __restrict__ part,
+                                     double beta, double* __restrict__ C, int64_t ldc) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= M * N) return;
+  const int64_t row = idx / N, col = idx % N;
+  double s = 0.0;
+  for (int z = 0; z < splits; ++z) s += part[z * M * N + idx];
+  double* c = C + row * ldc + col;
+  *c = (beta == 0.0) ? alpha * s : alpha * s + beta * (*c);
+}
+
+int gemm_splits(int64_t M, int64_t N, int64_t K) {
+  const int64_t tiles = pamd_cdiv(M, GBM) * pamd_cdiv(N, GBN);
+  const int64_t target = 1024;  // workgroups to keep 256 CUs busy
+  if (tiles >= target / 2 || K < 4 * 256) return 1;
+  int64_t s = pamd_cdiv(target, tiles);
+  s = min(s, K / 256);  // keep >= 256 K per slice
+  return (int)imax64(1, imin64(s, 64));
+}
+
+// ---------------------------------------------------------------------------
+// GEMV
+// ---------------------------------------------------------------------------
+
+// y[i] = alpha * A[i,:] x + beta * y[i]; one wavefront per row.
+__global__ __launch_bounds__(256) void gemv_n_kernel(int64_t m, int64_t n, double alpha,
+                                                     const double* __restrict__ A, int64_t lda,
+                                                     const double* __restrict__ x, double beta,
+                                                     double* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= m) return;
+  const double* a = A + row * lda;
+  double s = 0.0;
+  for (int64_t j = lane; j < n; j += 64) s += a[j] * x[j];
+  s = wave_sum(s);
+  if (lane == 0) y[row] = (beta == 0.0) ? alpha * s : alpha * s + beta * y[row];
+}
+
+// Column block of 64 (lanes) x row slice (grid.y); 4 waves split the rows.
+// With part != NULL, slice sums go to part[slice * n + col].
+__global__ __launch_bounds__(256) void gemv_t_kernel(int64_t m, int64_t n, int64_t rows_per_slice,
+                                                     double alpha, const double* __restrict__ A, int64_t lda,
+                                                     const double* __restrict__ x, double beta,
+                                                     double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t col = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t r0 = (int64_t)blockIdx.y * rows_per_slice;
+  const int64_t r1 = min(m, r0 + rows_per_slice);
+  double s = 0.0;
+  if (col < n)
+    for (int64_t i = r0 + wave; i < r1; i += 4) s += A[i * lda + col] * x[i];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && col < n) {
+    const double t = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    if (part)
+      part[(int64_t)blockIdx.y * n + col] = t;
+    else
+      y[col] = (beta == 0.0) ? alpha * t : alpha * t + beta * y[col];
+  }
+}
+
+__global__ void gemv_t_reduce_kernel(int64_t n, int slices, double alpha, const double* __restrict__ part,
+                                     double beta, double* __restrict__ y) {
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n) return;
+  double s = 0.0;
+  for (int z = 0; z < slices; ++z) s += part[(int64_t)z * n + col];
+  y[col] = (beta == 0.0) ? alpha * s : alpha * s + beta * y[col];
+}
+
+int gemv_t_slices(int64_t m, int64_t n) {
+  const int64_t cb = pamd_cdiv(n, 64);
+  int64_t s = pamd_cdiv(1024, cb);
+  s = imin64(s, pamd_cdiv(m, 256));
+  return (int)imax64(1, imin64(s, 1024));
+}
+
+__global__ __launch_bounds__(1024) void nrm2_kernel(int64_t n, const double* __restrict__ x,
+                                                    double* __restrict__ out) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += x[i] * x[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[0] = sqrt(s);
+}
+
+__global__ __launch_bounds__(1024) void trsv_kernel(int trans, int n, const double* __restrict__ R, int64_t ldr,
+                                                    double* __restrict__ x) {
+  extern __shared__ double smem[];
+  double* xs = smem;  // n
+  for (int i = threadIdx.x; i < n; i += blockDim.x) xs[i] = x[i];
+  __syncthreads();
+  trsv_upper_block(trans, n, R, ldr, xs, smem + n);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = xs[i];
+}
+
+}  // namespace
+
+extern "C" int pamd_gemm_workspace(int64_t M, int64_t N, int64_t K, int64_t* bytes) {
+  PAMD_ARG_CHECK(bytes != nullptr, 4);
+  const int s = gemm_splits(M, N, K);
+  *bytes = (s > 1) ? (int64_t)s * M * N * (int64_t)sizeof(double) : 0;
+  return PAMD_OK;
+}
+
+extern "C" int pamd_gemm_f64(int transA, int transB, int64_t M, int64_t N, int64_t K, double alpha,
+                             const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                             int64_t ldc, void* workspace, int64_t workspace_bytes, void* stream) {
+  PAMD_ARG_CHECK(transA == 0 || transA == 1, 1);
+  PAMD_ARG_CHECK(transB == 0 || transB == 1, 2);
+  PAMD_ARG_CHECK(M >= 0, 3);
+  PAMD_ARG_CHECK(N >= 0, 4);
+  PAMD_ARG_CHECK(K >= 0, 5);
+  PAMD_ARG_CHECK(A != nullptr || M * K == 0, 7);
+  PAMD_ARG_CHECK(lda >= (transA ? M : K) && lda >= 1, 8);
+  PAMD_ARG_CHECK(B != nullptr || K * N == 0, 9);
+  PAMD_ARG_CHECK(ldb >= (transB ? K : N) && ldb >= 1, 10);
+  PAMD_ARG_CHECK(C != nullptr || M * N == 0, 12);
+  PAMD_ARG_CHECK(ldc >= N && ldc >= 1, 13);
+  if (M == 0 || N == 0) return PAMD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  int splits = gemm_splits(M, N, K);
+  if (splits > 1 && (workspace == nullptr || workspace_bytes < (int64_t)splits * M * N * (int64_t)sizeof(double)))
+    splits = 1;
+  const int64_t kchunk = splits > 1 ? pamd_cdiv(pamd_cdiv(K, splits), GBK) * GBK : imax64(K, 1);
+  if (splits > 1) splits = (int)pamd_cdiv(K, kchunk);
+  double* part = splits > 1 ? (double*)workspace : nullptr;
+  dim3 grid((unsigned)pamd_cdiv(N, GBN), (unsigned)pamd_cdiv(M, GBM), (unsigned)splits);
+  if (transA == 0 && transB == 0)
+    gemm_f64_kernel<0, 0><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part);
+  else if (transA == 0 && transB == 1)
+    gemm_f64_kernel<0, 1><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part);
+  else if (transA == 1 && transB == 0)
+    gemm_f64_kernel<1, 0><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part);
+  else
+    gemm_f64_kernel<1, 1><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part);
+  PAMD_LAUNCH_CHECK();
+  if (splits > 1) {
+    const int64_t tot = M * N;
+    splitk_reduce_kernel<<<(unsigned)pamd_cdiv(tot, 256), 256, 0, st>>>(M, N, splits, alpha, part, beta, C, ldc);
+    PAMD_LAUNCH_CHECK();
+  }
+  return PAMD_OK;
+}
+
+extern "C" int pamd_gemv_workspace(int trans, int64_t m, int64_t n, int64_t* bytes) {
+  PAMD_ARG_CHECK(bytes != nullptr, 4);
+  *bytes = 0;
+  if (trans == 1) {
+    const int s = gemv_t_slices(m, n);
+    if (s > 1) *bytes = (int64_t)s * n * (int64_t)sizeof(double);
+  }
+  return PAMD_OK;
+}
+
+extern "C" int pamd_gemv_f64(int trans, int64_t m, int64_t n, double alpha, const double* A, int64_t lda,
+                             const double* x, double beta, double* y, void* workspace, int64_t workspace_bytes,
+                             void* stream) {
+  PAMD_ARG_CHECK(trans == 0 || trans == 1, 1);
+  PAMD_ARG_CHECK(m >= 0, 2);
+  PAMD_ARG_CHECK(n >= 0, 3);
+  PAMD_ARG_CHECK(A != nullptr || m * n == 0, 5);
+  PAMD_ARG_CHECK(lda >= n && lda >= 1, 6);
+  hipStream_t st = (hipStream_t)stream;
+  if (trans == 0) {
+    if (m == 0) return PAMD_OK;
+    gemv_n_kernel<<<(unsigned)pamd_cdiv(m, 4), 256, 0, st>>>(m, n, alpha, A, lda, x, beta, y);
+    PAMD_LAUNCH_CHECK();
+    return PAMD_OK;
+  }
+  if (n == 0) return PAMD_OK;
+  int slices = gemv_t_slices(m, n);
+  if (slices > 1 && (workspace == nullptr || workspace_bytes < (int64_t)slices * n * (int64_t)sizeof(double)))
+    slices = 1;
+  const int64_t rps = imax64(1, pamd_cdiv(m, slices));
+  slices = (int)imax64(1, pamd_cdiv(m, rps));
+  double* part = slices > 1 ? (double*)workspace : nullptr;
+  dim3 grid((unsigned)pamd_cdiv(n, 64), (unsigned)slices);
+  gemv_t_kernel<<<grid, 256, 0, st>>>(m, n, rps, alpha, A, lda, x, beta, y, part);
+  PAMD_LAUNCH_CHECK();
+  if (part) {
+    gemv_t_reduce_kernel<<<(unsigned)pamd_cdiv(n, 256), 256, 0, st>>>(n, slices, alpha, part, beta, y);
+    PAMD_LAUNCH_CHECK();
+  }
+  return PAMD_OK;
+}
+
+extern "C" int pamd_nrm2_f64(int64_t n, const double* x, double* result, void* stream) {
+  PAMD_ARG_CHECK(n >= 0, 1);
+  PAMD_ARG_CHECK(x != nullptr || n == 0, 2);
+  PAMD_ARG_CHECK(result != nullptr, 3);
+  nrm2_kernel<<<1, 1024, 0, (hipStream_t)stream>>>(n, x, result);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_trsv_upper_f64(int trans, int64_t n, const double* R, int64_t ldr, double* x, void* stream) {
+  PAMD_ARG_CHECK(trans == 0 || trans == 1, 1);
+  PAMD_ARG_CHECK(n >= 0 && n <= PAMD_TRSV_MAX_N, 2);
+  PAMD_ARG_CHECK(R != nullptr || n == 0, 3);
+  PAMD_ARG_CHECK(ldr >= n && ldr >= 1, 4);
+  PAMD_ARG_CHECK(x != nullptr || n == 0, 5);
+  if (n == 0) return PAMD_OK;
+  const size_t shmem = ((size_t)n + TRSV_SCRATCH_DOUBLES) * sizeof(double);
+  trsv_kernel<<<1, 1024, shmem, (hipStream_t)stream>>>(trans, (int)n, R, ldr, x);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}

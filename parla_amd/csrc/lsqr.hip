@@ -1,0 +1,616 @@
+// Preconditioned LSQR with every recurrence on the device.
+//
+// Operator: A_pc = [A; sd I] R^{-1} (sd = sqrt(delta), R upper triangular, or
+// no R).  One LSQR iteration is three kernels:
+//
+//   lsqr_pass    (multi-workgroup, one read of A): for each row i
+//                  u_i <- c1 * (A w)_i + c2 * u_i,   y += u_i * A[i, :]
+//                where w = R^{-1} v.  The bidiagonalisation needs A_pc v and
+//                then A_pc^T applied to the *normalised* new u; since
+//                A^T (u / beta) = (A^T u) / beta, both products come out of the
+//                same pass over A: HBM traffic is one read of A per iteration
+//                instead of two.
+//   lsqr_reduce  (one workgroup per 64 columns): y = sum of the per-workgroup
+//                partials in fixed order, plus the lifted rows sd * I.
+//   lsqr_step    (one 1024-thread workgroup): beta, v <- R^{-T} y / beta - beta v,
+//                alpha, the Givens recurrences and stopping tests of the PARLA
+//                lsqr (comps/determiter/lsqr.py:412-526), x / w updates, and
+//                w_next = R^{-1} v for the next pass.  It sets `done` when a
+//                stopping test fires; every kernel returns immediately once
+//                `done` is set, so the host can enqueue iterations in batches
+//                and read the state back once per batch.
+//
+// u is kept unnormalised: the 1/beta of lsqr.py:425 is folded into the next
+// pass's c2 = -alpha / beta.
+#include "common.h"
+#include "trsv_dev.h"
+
+namespace {
+
+struct LsqrState {
+  double alfa, beta, anorm, acond, ddnorm, res2, xnorm, xxnorm, z, cs2, sn2;
+  double rhobar, phibar, rnorm, r1norm, r2norm, arnorm, bnorm;
+  double c1, c2, ssq_new, ssq_in;
+  int64_t itn, istop, done, early;
+};
+
+struct LsqrConst {
+  int64_t m, n, lda, ldr, iter_lim;
+  double sd, damp, atol, btol, ctol;
+  int has_r, calc_var, has_x0, grid;
+};
+
+constexpr int PASS_THREADS = 256;
+
+__host__ __device__ inline int lsqr_grid(int64_t m) { return (int)imax64(1, imin64(1024, pamd_cdiv(m, 64))); }
+
+struct LsqrLayout {
+  int64_t st, u, v, w, work, y, pv, pb, pl, total;
+};
+
+__host__ __device__ inline LsqrLayout lsqr_layout(int64_t m, int64_t n) {
+  LsqrLayout L;
+  const int G = lsqr_grid(m);
+  const int64_t nblk = pamd_cdiv(imax64(n, 1), 64);
+  L.st = 0;
+  L.u = 64;
+  L.v = L.u + m + n;
+  L.w = L.v + n;
+  L.work = L.w + n;
+  L.y = L.work + n;
+  L.pv = (L.y + n + 31) / 32 * 32;
+  L.pb = L.pv + (int64_t)G * n;
+  L.pl = L.pb + (int64_t)G * 2;
+  L.total = L.pl + nblk * 2 + 32;
+  return L;
+}
+
+// ---------------------------------------------------------------------------
+// pass over A
+// ---------------------------------------------------------------------------
+// NPL = columns per lane held in registers (n <= 64 * NPL); NPL = 0: any n,
+// the row is read twice (second read hits cache).
+template <int NPL>
+__global__ __launch_bounds__(PASS_THREADS) void lsqr_pass_kernel(LsqrConst K, const double* __restrict__ A,
+                                                                 double* __restrict__ ws) {
+  LsqrState* S = reinterpret_cast<LsqrState*>(ws);
+  if (S->done) return;
+  const LsqrLayout L = lsqr_layout(K.m, K.n);
+  const double* __restrict__ w = ws + L.work;
+  double* __restrict__ u = ws + L.u;
+  double* __restrict__ pv = ws + L.pv;
+  double* __restrict__ pb = ws + L.pb;
+  const double c1 = S->c1, c2 = S->c2;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t n = K.n;
+  const int64_t gw = (int64_t)gridDim.x * (PASS_THREADS / 64);
+  double ssq_new = 0.0, ssq_in = 0.0;
+
+  __shared__ double red[PASS_THREADS / 64][64 * (NPL > 0 ? NPL : 1)];
+  __shared__ double sred[PASS_THREADS / 64][2];
+
+  if constexpr (NPL > 0) {
+    double wr[NPL], acc[NPL];
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      const int64_t c = lane + 64 * q;
+      wr[q] = (c < n) ? w[c] : 0.0;
+      acc[q] = 0.0;
+    }
+    int64_t row = (int64_t)blockIdx.x * (PASS_THREADS / 64) + wave;
+    double a[NPL];
+    if (row < K.m) {
+#pragma unroll
+      for (int q = 0; q < NPL; ++q) {
+        const int64_t c = lane + 64 * q;
+        a[q] = (c < n) ? A[row * K.lda + c] : 0.0;
+      }
+    }
+    for (; row < K.m; row += gw) {
+      // prefetch the next row while this one is reduced
+      const int64_t nrow = row + gw;
+      double an[NPL];
+      if (nrow < K.m) {
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) {
+          const int64_t c = lane + 64 * q;
+          an[q] = (c < n) ? A[nrow * K.lda + c] : 0.0;
+        }
+      }
+      const double uin = u[row];
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < NPL; ++q) s += a[q] * wr[q];
+      s = wave_sum(s);
+      const double un = c1 * s + c2 * uin;
+#pragma unroll
+      for (int q = 0; q < NPL; ++q) acc[q] += un * a[q];
+      ssq_new += un * un;
+      ssq_in += uin * uin;
+      if (lane == 0) u[row] = un;
+      if (nrow < K.m) {
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) a[q] = an[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) red[wave][lane + 64 * q] = acc[q];
+    if (lane == 0) {
+      sred[wave][0] = ssq_new;
+      sred[wave][1] = ssq_in;
+    }
+    __syncthreads();
+    for (int64_t c = threadIdx.x; c < n; c += PASS_THREADS) {
+      double t = 0.0;
+#pragma unroll
+      for (int wv = 0; wv < PASS_THREADS / 64; ++wv) t += red[wv][c];
+      pv[(int64_t)blockIdx.x * n + c] = t;
+    }
+  } else {
+    // generic n: the workgroup owns rows; threads stride over columns
+    for (int64_t c = threadIdx.x; c < n; c += PASS_THREADS) pv[(int64_t)blockIdx.x * n + c] = 0.0;
+    __syncthreads();
+    // uniform trip count across the block (the loop body has barriers)
+    for (int64_t base = (int64_t)blockIdx.x * (PASS_THREADS / 64); base < K.m; base += gw) {
+      const int64_t row = base + wave;
+      const bool valid = row < K.m;
+      const double* ar = A + (valid ? row : 0) * K.lda;
+      double un = 0.0;
+      if (valid) {
+        double s = 0.0;
+        for (int64_t c = lane; c < n; c += 64) s += ar[c] * w[c];
+        s = wave_sum(s);
+        const double uin = u[row];
+        un = c1 * s + c2 * uin;
+        ssq_new += un * un;
+        ssq_in += uin * uin;
+        if (lane == 0) u[row] = un;
+      }
+      // accumulate into this block's partial, one wave at a time (fixed order)
+      for (int wv = 0; wv < PASS_THREADS / 64; ++wv) {
+        __syncthreads();
+        if (wv == wave && valid)
+          for (int64_t c = lane; c < n; c += 64) pv[(int64_t)blockIdx.x * n + c] += un * ar[c];
+      }
+    }
+    if (lane == 0) {
+      sred[wave][0] = ssq_new;
+      sred[wave][1] = ssq_in;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double a0 = 0.0, a1 = 0.0;
+    for (int wv = 0; wv < PASS_THREADS / 64; ++wv) {
+      a0 += sred[wv][0];
+      a1 += sred[wv][1];
+    }
+    pb[2 * blockIdx.x] = a0;
+    pb[2 * blockIdx.x + 1] = a1;
+  }
+}
+
+// y[c] = sum_g pv[g][c] (+ lifted rows); block b also writes pl[b] partial sums
+__global__ __launch_bounds__(256) void lsqr_reduce_kernel(LsqrConst K, double* __restrict__ ws) {
+  LsqrState* S = reinterpret_cast<LsqrState*>(ws);
+  if (S->done) return;
+  const LsqrLayout L = lsqr_layout(K.m, K.n);
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
+  const double* pv = ws + L.pv;
+  double s = 0.0;
+  if (c < K.n)
+    for (int g = wave; g < K.grid; g += 4) s += pv[(int64_t)g * K.n + c];
+  red[wave][lane] = s;
+  __syncthreads();
+  double lift_new = 0.0, lift_in = 0.0;
+  if (wave == 0 && c < K.n) {
+    double t = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    if (K.sd > 0.0) {
+      double* ul = ws + L.u + K.m;
+      const double uin = ul[c];
+      const double un = S->c1 * (K.sd * ws[L.work + c]) + S->c2 * uin;
+      ul[c] = un;
+      t += K.sd * un;
+      lift_new = un * un;
+      lift_in = uin * uin;
+    }
+    ws[L.y + c] = t;
+  }
+  if (wave == 0) {
+    lift_new = wave_sum(lift_new);
+    lift_in = wave_sum(lift_in);
+    if (lane == 0) {
+      ws[L.pl + 2 * blockIdx.x] = lift_new;
+      ws[L.pl + 2 * blockIdx.x + 1] = lift_in;
+    }
+  }
+}
+
+__device__ inline double dsign(double a) { return (a > 0.0) ? 1.0 : ((a < 0.0) ? -1.0 : 0.0); }
+
+// Stable Givens rotation of comps/determiter/lsqr.py:63-95.
+__device__ inline void sym_ortho(double a, double b, double& c, double& s, double& r) {
+  if (b == 0.0) {
+    c = dsign(a);
+    s = 0.0;
+    r = fabs(a);
+  } else if (a == 0.0) {
+    c = 0.0;
+    s = dsign(b);
+    r = fabs(b);
+  } else if (fabs(b) > fabs(a)) {
+    const double tau = a / b;
+    s = dsign(b) / sqrt(1.0 + tau * tau);
+    c = s * tau;
+    r = b / s;
+  } else {
+    const double tau = b / a;
+    c = dsign(a) / sqrt(1.0 + tau * tau);
+    s = c * tau;
+    r = a / c;
+  }
+}
+
+// Sum of the pass partials (fixed order).  All threads get the totals.
+__device__ inline void total_ssq(const LsqrConst& K, const double* ws, const LsqrLayout& L, double* red,
+                                 double& ssq_new, double& ssq_in) {
+  double a = 0.0, b = 0.0;
+  for (int g = threadIdx.x; g < K.grid; g += blockDim.x) {
+    a += ws[L.pb + 2 * g];
+    b += ws[L.pb + 2 * g + 1];
+  }
+  const int nblk = (int)pamd_cdiv(imax64(K.n, 1), 64);
+  if (K.sd > 0.0)
+    for (int q = threadIdx.x; q < nblk; q += blockDim.x) {
+      a += ws[L.pl + 2 * q];
+      b += ws[L.pl + 2 * q + 1];
+    }
+  ssq_new = block_sum(a, red);
+  ssq_in = block_sum(b, red);
+}
+
+// v_out = R^{-T}(y * inv) + coef * v_in (LDS staging for the triangular solve).
+// Returns ||v_out||^2.  v_in may alias v_out.
+__device__ inline double apply_adjoint(const LsqrConst& K, const double* __restrict__ R, const double* y,
+                                       double inv, const double* v_in, double coef, double* v_out, double* xs,
+                                       double* scratch, double* red) {
+  const int64_t n = K.n;
+  double ss = 0.0;
+  if (K.has_r) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) xs[i] = y[i] * inv;
+    __syncthreads();
+    trsv_upper_block(1, (int)n, R, K.ldr, xs, scratch);
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const double t = (coef != 0.0) ? xs[i] + coef * v_in[i] : xs[i];
+      v_out[i] = t;
+      ss += t * t;
+    }
+  } else {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const double yi = y[i] * inv;
+      const double t = (coef != 0.0) ? yi + coef * v_in[i] : yi;
+      v_out[i] = t;
+      ss += t * t;
+    }
+  }
+  return block_sum(ss, red);
+}
+
+// work = R^{-1} v (or v)
+__device__ inline void apply_forward(const LsqrConst& K, const double* __restrict__ R, const double* v, double* work,
+                                     double* xs, double* scratch) {
+  const int64_t n = K.n;
+  if (K.has_r) {
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) xs[i] = v[i];
+    __syncthreads();
+    trsv_upper_block(0, (int)n, R, K.ldr, xs, scratch);
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) work[i] = xs[i];
+  } else {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) work[i] = v[i];
+  }
+}
+
+// Before the first pass: state, x = x0 or 0, u = b (lifted part 0),
+// work = R^{-1} x0 with (c1, c2) = (-1, 1), or work = 0 with (c1, c2) = (0, 1).
+__global__ __launch_bounds__(1024) void lsqr_init_kernel(LsqrConst K, const double* __restrict__ R,
+                                                         const double* __restrict__ b, const double* __restrict__ x0,
+                                                         double* __restrict__ x, double* __restrict__ var,
+                                                         double* __restrict__ ws) {
+  extern __shared__ double smem[];
+  const LsqrLayout L = lsqr_layout(K.m, K.n);
+  LsqrState* S = reinterpret_cast<LsqrState*>(ws);
+  double* u = ws + L.u;
+  for (int64_t i = threadIdx.x; i < K.m; i += blockDim.x) u[i] = b[i];
+  for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) {
+    u[K.m + i] = 0.0;
+    x[i] = K.has_x0 ? x0[i] : 0.0;
+    if (var) var[i] = 0.0;
+  }
+  if (K.has_x0) {
+    apply_forward(K, R, x0, ws + L.work, smem, smem + K.n);
+  } else {
+    for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) ws[L.work + i] = 0.0;
+  }
+  if (threadIdx.x == 0) {
+    LsqrState s = {};
+    s.cs2 = -1.0;
+    s.c1 = K.has_x0 ? -1.0 : 0.0;
+    s.c2 = 1.0;
+    *S = s;
+  }
+}
+
+// After the first pass (lsqr.py:362-395): beta, v, alfa, w and the early exit.
+__global__ __launch_bounds__(1024) void lsqr_setup_kernel(LsqrConst K, const double* __restrict__ R,
+                                                          double* __restrict__ x, double* __restrict__ ws) {
+  extern __shared__ double smem[];
+  __shared__ double red[16];
+  const LsqrLayout L = lsqr_layout(K.m, K.n);
+  LsqrState* S = reinterpret_cast<LsqrState*>(ws);
+  double* xs = smem;
+  double* scratch = smem + K.n;
+  double ssq_new, ssq_in;
+  total_ssq(K, ws, L, red, ssq_new, ssq_in);
+  const double bnorm = sqrt(ssq_in);
+  const double beta = sqrt(ssq_new);
+  double* v = ws + L.v;
+  double alfa = 0.0;
+  if (beta > 0.0) {
+    alfa = sqrt(apply_adjoint(K, R, ws + L.y, 1.0 / beta, v, 0.0, v, xs, scratch, red));
+  } else {
+    for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) v[i] = x[i];
+  }
+  if (alfa > 0.0) {
+    const double ia = 1.0 / alfa;
+    for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) v[i] = ia * v[i];
+  }
+  for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) ws[L.w + i] = v[i];
+  const double arnorm = alfa * beta;
+  const bool stop = (arnorm == 0.0) || (K.iter_lim <= 0);
+  if (!stop) apply_forward(K, R, v, ws + L.work, xs, scratch);
+  if (threadIdx.x == 0) {
+    S->bnorm = bnorm;
+    S->beta = beta;
+    S->alfa = alfa;
+    S->rhobar = alfa;
+    S->phibar = beta;
+    S->rnorm = S->r1norm = S->r2norm = beta;
+    S->arnorm = arnorm;
+    S->istop = 0;
+    S->itn = 0;
+    S->early = (arnorm == 0.0) ? 1 : 0;
+    S->done = stop ? 1 : 0;
+    S->c1 = 1.0;
+    S->c2 = -alfa * (1.0 / beta);
+  }
+}
+
+// One LSQR iteration after the pass (lsqr.py:413-526).
+__global__ __launch_bounds__(1024) void lsqr_step_kernel(LsqrConst K, const double* __restrict__ R,
+                                                         double* __restrict__ x, double* __restrict__ var,
+                                                         double* __restrict__ arnorms, double* __restrict__ ws) {
+  LsqrState* S = reinterpret_cast<LsqrState*>(ws);
+  if (S->done) return;
+  extern __shared__ double smem[];
+  __shared__ double red[16];
+  __shared__ double bc[8];
+  const LsqrLayout L = lsqr_layout(K.m, K.n);
+  double* xs = smem;
+  double* scratch = smem + K.n;
+  double* v = ws + L.v;
+  double* w = ws + L.w;
+  const int64_t n = K.n;
+  const double eps = 2.220446049250313e-16;
+
+  double ssq_new, ssq_in;
+  total_ssq(K, ws, L, red, ssq_new, ssq_in);
+  const double beta = sqrt(ssq_new);
+  double alfa = S->alfa;
+  double anorm = S->anorm;
+  if (beta > 0.0) {
+    anorm = sqrt(anorm * anorm + alfa * alfa + beta * beta + K.damp * K.damp);
+    const double a2 = apply_adjoint(K, R, ws + L.y, 1.0 / beta, v, -beta, v, xs, scratch, red);
+    alfa = sqrt(a2);
+    if (alfa > 0.0) {
+      const double ia = 1.0 / alfa;
+      for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = ia * v[i];
+    }
+  }
+  // scalar rotations (identical on every thread; no broadcast needed)
+  const double damp = K.damp;
+  const double rhobar0 = S->rhobar, phibar0 = S->phibar;
+  const double rhobar1 = sqrt(rhobar0 * rhobar0 + damp * damp);
+  const double cs1 = rhobar0 / rhobar1;
+  const double sn1 = damp / rhobar1;
+  const double psi = sn1 * phibar0;
+  double phibar = cs1 * phibar0;
+  double cs, sn, rho;
+  sym_ortho(rhobar1, beta, cs, sn, rho);
+  const double theta = sn * alfa;
+  const double rhobar = -cs * alfa;
+  const double phi = cs * phibar;
+  phibar = sn * phibar;
+  const double tau = sn * phi;
+  const double t1 = phi / rho;
+  const double t2 = -theta / rho;
+  const double irho = 1.0 / rho;
+  __syncthreads();  // v complete before w = v + t2 w
+  double dd = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double wi = w[i];
+    const double dk = irho * wi;
+    dd += dk * dk;
+    if (K.calc_var) var[i] += dk * dk;
+    x[i] = x[i] + t1 * wi;
+    w[i] = v[i] + t2 * wi;
+  }
+  const double ddsum = block_sum(dd, red);
+  if (threadIdx.x == 0) {
+    const int64_t itn_prev = S->itn;
+    arnorms[itn_prev] = S->arnorm;  // logged at the top of the iteration (lsqr.py:413)
+    const int64_t itn = itn_prev + 1;
+    const double ddnorm = S->ddnorm + ddsum;
+    const double delta = S->sn2 * rho;
+    const double gambar = -S->cs2 * rho;
+    const double rhs = phi - delta * S->z;
+    const double zbar = rhs / gambar;
+    const double xnorm = sqrt(S->xxnorm + zbar * zbar);
+    const double gamma = sqrt(gambar * gambar + theta * theta);
+    const double cs2 = gambar / gamma;
+    const double sn2 = theta / gamma;
+    const double z = rhs / gamma;
+    const double xxnorm = S->xxnorm + z * z;
+    const double acond = anorm * sqrt(ddnorm);
+    const double res1 = phibar * phibar;
+    const double res2 = S->res2 + psi * psi;
+    const double rnorm = sqrt(res1 + res2);
+    const double arnorm = alfa * fabs(tau);
+    const double r1sq = rnorm * rnorm - damp * damp * xxnorm;
+    double r1norm = sqrt(fabs(r1sq));
+    if (r1sq < 0) r1norm = -r1norm;
+    const double bnorm = S->bnorm;
+    const double test1 = rnorm / bnorm;
+    const double test2 = arnorm / (anorm * rnorm + eps);
+    const double test3 = 1.0 / (acond + eps);
+    const double tt1 = test1 / (1.0 + anorm * xnorm / bnorm);
+    const double rtol = K.btol + K.atol * anorm * xnorm / bnorm;
+    int64_t istop = 0;
+    if (itn >= K.iter_lim) istop = 7;
+    if (1.0 + test3 <= 1.0) istop = 6;
+    if (1.0 + test2 <= 1.0) istop = 5;
+    if (1.0 + tt1 <= 1.0) istop = 4;
+    if (test3 <= K.ctol) istop = 3;
+    if (test2 <= K.atol) istop = 2;
+    if (test1 <= rtol) istop = 1;
+    S->itn = itn;
+    S->istop = istop;
+    S->alfa = alfa;
+    S->beta = beta;
+    S->anorm = anorm;
+    S->acond = acond;
+    S->ddnorm = ddnorm;
+    S->res2 = res2;
+    S->xnorm = xnorm;
+    S->xxnorm = xxnorm;
+    S->z = z;
+    S->cs2 = cs2;
+    S->sn2 = sn2;
+    S->rhobar = rhobar;
+    S->phibar = phibar;
+    S->rnorm = rnorm;
+    S->r1norm = r1norm;
+    S->r2norm = rnorm;
+    S->arnorm = arnorm;
+    S->c1 = 1.0;
+    S->c2 = (beta > 0.0) ? -alfa * (1.0 / beta) : -alfa;
+    bc[0] = (istop != 0) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  if (bc[0] == 0.0) {
+    apply_forward(K, R, v, ws + L.work, xs, scratch);
+  }
+  if (threadIdx.x == 0 && bc[0] != 0.0) S->done = 1;
+}
+
+int launch_pass(const LsqrConst& K, const double* A, double* ws, hipStream_t st) {
+  const int64_t n = K.n;
+  if (n <= 64)
+    lsqr_pass_kernel<1><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
+  else if (n <= 128)
+    lsqr_pass_kernel<2><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
+  else if (n <= 256)
+    lsqr_pass_kernel<4><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
+  else if (n <= 512)
+    lsqr_pass_kernel<8><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
+  else if (n <= 1024)
+    lsqr_pass_kernel<16><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
+  else
+    lsqr_pass_kernel<0><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
+  PAMD_LAUNCH_CHECK();
+  lsqr_reduce_kernel<<<(unsigned)pamd_cdiv(n, 64), 256, 0, st>>>(K, ws);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int pamd_lsqr_workspace(int64_t m, int64_t n, int64_t iter_lim, int64_t* bytes) {
+  PAMD_ARG_CHECK(m >= 0, 1);
+  PAMD_ARG_CHECK(n >= 1, 2);
+  PAMD_ARG_CHECK(bytes != nullptr, 4);
+  (void)iter_lim;
+  *bytes = lsqr_layout(m, n).total * (int64_t)sizeof(double);
+  return PAMD_OK;
+}
+
+extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, double* var, double* arnorms,
+                                 pamd_lsqr_result* result, void* workspace, void* stream) {
+  PAMD_ARG_CHECK(prob != nullptr, 1);
+  const pamd_lsqr_problem& P = *prob;
+  PAMD_ARG_CHECK(P.m >= 0 && P.n >= 1, 1);
+  PAMD_ARG_CHECK(P.A != nullptr && P.lda >= P.n, 1);
+  PAMD_ARG_CHECK(P.b != nullptr, 1);
+  PAMD_ARG_CHECK(P.R == nullptr || (P.ldr >= P.n && P.n <= PAMD_TRSV_MAX_N), 1);
+  PAMD_ARG_CHECK(x != nullptr, 2);
+  PAMD_ARG_CHECK(var != nullptr || !P.calc_var, 3);
+  PAMD_ARG_CHECK(arnorms != nullptr || P.iter_lim <= 0, 4);
+  PAMD_ARG_CHECK(result != nullptr, 5);
+  PAMD_ARG_CHECK(workspace != nullptr, 6);
+  hipStream_t st = (hipStream_t)stream;
+  LsqrConst K;
+  K.m = P.m;
+  K.n = P.n;
+  K.lda = P.lda;
+  K.ldr = P.ldr;
+  K.iter_lim = P.iter_lim;
+  K.sd = P.sqrt_delta > 0 ? P.sqrt_delta : 0.0;
+  K.damp = P.damp;
+  K.atol = P.atol;
+  K.btol = P.btol;
+  K.ctol = (P.conlim > 0) ? 1.0 / P.conlim : 0.0;
+  K.has_r = P.R != nullptr;
+  K.calc_var = P.calc_var != 0;
+  K.has_x0 = P.x0 != nullptr;
+  K.grid = lsqr_grid(P.m);
+  double* ws = (double*)workspace;
+  const size_t shmem = (size_t)(K.has_r ? (P.n + TRSV_SCRATCH_DOUBLES) : 1) * sizeof(double);
+
+  lsqr_init_kernel<<<1, 1024, shmem, st>>>(K, P.R, P.b, P.x0, x, var, ws);
+  PAMD_LAUNCH_CHECK();
+  int rc = launch_pass(K, P.A, ws, st);
+  if (rc) return rc;
+  lsqr_setup_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, ws);
+  PAMD_LAUNCH_CHECK();
+
+  LsqrState hs;
+  int64_t batch = 4;
+  int64_t launched = 0;
+  for (;;) {
+    PAMD_HIP_CHECK(hipMemcpyAsync(&hs, ws, sizeof(LsqrState), hipMemcpyDeviceToHost, st));
+    PAMD_HIP_CHECK(hipStreamSynchronize(st));
+    if (hs.done || launched >= P.iter_lim) break;
+    const int64_t todo = imin64(batch, P.iter_lim - launched);
+    for (int64_t it = 0; it < todo; ++it) {
+      rc = launch_pass(K, P.A, ws, st);
+      if (rc) return rc;
+      lsqr_step_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, var, arnorms, ws);
+      PAMD_LAUNCH_CHECK();
+    }
+    launched += todo;
+    batch = imin64(batch * 2, 64);
+  }
+  result->istop = hs.istop;
+  result->itn = hs.itn;
+  result->r1norm = hs.r1norm;
+  result->r2norm = hs.r2norm;
+  result->anorm = hs.anorm;
+  result->acond = hs.acond;
+  result->arnorm = hs.arnorm;
+  result->xnorm = hs.xnorm;
+  result->n_log = hs.early ? -1 : hs.itn;
+  return PAMD_OK;
+}

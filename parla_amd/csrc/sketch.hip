@@ -1,0 +1,417 @@
+// Sketching operators: counter-based Gaussian fill, SJLT draw, deterministic
+// CSC -> CSR transpose, and the sparse sketch applications S @ A and A @ S.
+//
+// The SJLT products are gathers over the CSR of the wide operator W, so every
+// output element is written once, with a fixed summation order (column index
+// ascending, the order in which scipy's CSC product accumulates), and without
+// atomics:
+//   left   out[i, :] = sum_{(j, v) in row i of W} v * A[j, :]     (W @ A)
+//   right  out[r, c] = sum_{(j, v) in row c of W} v * A[r, j]     (A @ W^T)
+// The right product stages a 16-row x 512-column tile of A in LDS and walks
+// each output column's CSR list through the tiles, so W's index arrays are
+// read once per 16 rows of A rather than once per row.
+#include "common.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) and Box-Muller.
+// ---------------------------------------------------------------------------
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo1 = 0xCD9E8D57u * c.z;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// 53-bit uniform in (0, 1]
+__device__ __forceinline__ double u53_open0(uint32_t hi, uint32_t lo) {
+  const uint64_t b = (((uint64_t)hi << 32) | lo) >> 11;
+  return ((double)b + 1.0) * (1.0 / 9007199254740992.0);
+}
+
+// Standard normal pair for counter `pair` (one Philox call -> two normals).
+__device__ __forceinline__ void normal_pair(uint64_t seed, uint64_t pair, double& z0, double& z1) {
+  const U4 r = philox4x32_10(U4{(uint32_t)pair, (uint32_t)(pair >> 32), 0x6A09E667u, 0u}, (uint32_t)seed,
+                             (uint32_t)(seed >> 32));
+  const double u1 = u53_open0(r.x, r.y);
+  const double u2 = u53_open0(r.z, r.w);
+  const double rad = sqrt(-2.0 * log(u1));
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  z0 = rad * c;
+  z1 = rad * s;
+}
+
+__global__ void gaussian_fill_kernel(int64_t rows, int64_t cols, double scale, uint64_t seed, uint64_t offset,
+                                     double* __restrict__ X, int64_t ld) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * cols) return;
+  const int64_t i = idx / cols, j = idx % cols;
+  const uint64_t e = offset + (uint64_t)idx;
+  double z0, z1;
+  normal_pair(seed, e >> 1, z0, z1);
+  X[i * ld + j] = scale * ((e & 1) ? z1 : z0);
+}
+
+// Column j: k distinct rows in [0, n_short) (rejection on repeats), signs.
+__global__ void sjlt_draw_kernel(int64_t n_short, int64_t n_long, int k, uint64_t seed, int32_t* __restrict__ rows,
+                                 double* __restrict__ vals) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_long) return;
+  const bool with_repl = n_short < k;
+  const double mag = 1.0 / sqrt((double)k);
+  int32_t picked[64];
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  for (int l = 0; l < k; ++l) {
+    int32_t r = 0;
+    uint32_t sign = 0;
+    bool placed = false;
+    for (uint32_t attempt = 0; attempt < 64 && !placed; ++attempt) {
+      const U4 u = philox4x32_10(U4{(uint32_t)j, (uint32_t)(j >> 32), (uint32_t)l, 0x510E527Fu + attempt}, k0, k1);
+      r = (int32_t)(((uint64_t)u.x * (uint64_t)n_short) >> 32);
+      sign = u.y & 1u;
+      bool dup = false;
+      if (!with_repl)
+        for (int q = 0; q < l; ++q) dup |= (picked[q] == r);
+      placed = !dup;
+    }
+    if (!placed) {  // 64 straight repeats (practically unreachable): first unused row
+      for (r = 0; r < (int32_t)n_short; ++r) {
+        bool dup = false;
+        for (int q = 0; q < l; ++q) dup |= (picked[q] == r);
+        if (!dup) break;
+      }
+    }
+    picked[l] = r;
+    vals[j * k + l] = sign ? -mag : mag;
+    rows[j * k + l] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CSC -> CSR (stable by column) in three passes over blocks of columns.
+// ---------------------------------------------------------------------------
+constexpr int CSR_CB = 256;  // columns per block
+
+__global__ __launch_bounds__(256) void csr_count_kernel(int64_t n_short, int64_t n_long, int k,
+                                                        const int32_t* __restrict__ rows, int32_t* __restrict__ cnt) {
+  extern __shared__ int32_t hist[];
+  for (int64_t r = threadIdx.x; r < n_short; r += blockDim.x) hist[r] = 0;
+  __syncthreads();
+  const int64_t j0 = (int64_t)blockIdx.x * CSR_CB;
+  const int64_t j1 = min(n_long, j0 + CSR_CB);
+  for (int64_t e = j0 * k + threadIdx.x; e < j1 * k; e += blockDim.x) atomicAdd(&hist[rows[e]], 1);
+  __syncthreads();
+  for (int64_t r = threadIdx.x; r < n_short; r += blockDim.x) cnt[(int64_t)blockIdx.x * n_short + r] = hist[r];
+}
+
+// rowptr = exclusive scan of row counts; cnt[b][r] <- start offset of block b in row r
+__global__ __launch_bounds__(1024) void csr_scan_kernel(int64_t n_short, int nblk, int32_t* __restrict__ cnt,
+                                                        int64_t* __restrict__ rowptr) {
+  __shared__ int64_t wsum[16];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t base = 0; base < n_short; base += blockDim.x) {
+    const int64_t r = base + threadIdx.x;
+    int64_t c = 0;
+    if (r < n_short)
+      for (int b = 0; b < nblk; ++b) c += cnt[(int64_t)b * n_short + r];
+    // inclusive scan inside the wave
+    int64_t v = c;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int64_t t = __shfl_up(v, off, 64);
+      if (lane >= off) v += t;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    int64_t wpre = 0;
+    for (int w = 0; w < wave; ++w) wpre += wsum[w];
+    const int64_t excl = carry + wpre + v - c;
+    if (r < n_short) {
+      rowptr[r] = excl;
+      int64_t run = excl;
+      for (int b = 0; b < nblk; ++b) {
+        const int32_t t = cnt[(int64_t)b * n_short + r];
+        cnt[(int64_t)b * n_short + r] = (int32_t)run;
+        run += t;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry = excl + c;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rowptr[n_short] = carry;
+}
+
+// Thread t owns rows r with r % 256 == t and places them in column order.
+__global__ __launch_bounds__(256) void csr_fill_kernel(int64_t n_short, int64_t n_long, int k,
+                                                       const int32_t* __restrict__ rows,
+                                                       const double* __restrict__ vals,
+                                                       const int32_t* __restrict__ cnt, int32_t* __restrict__ cols,
+                                                       double* __restrict__ cvals) {
+  extern __shared__ int32_t lrows[];  // CSR_CB * k
+  const int64_t j0 = (int64_t)blockIdx.x * CSR_CB;
+  const int64_t j1 = min(n_long, j0 + CSR_CB);
+  const int64_t ne = (j1 - j0) * k;
+  for (int64_t e = threadIdx.x; e < ne; e += blockDim.x) lrows[e] = rows[j0 * k + e];
+  __syncthreads();
+  const int32_t* off = cnt + (int64_t)blockIdx.x * n_short;
+  // per-thread running positions for up to 64 owned rows
+  for (int64_t rbase = threadIdx.x; rbase < n_short; rbase += 256 * 64) {
+    int32_t pos[64];
+    int nown = 0;
+    for (int64_t r = rbase; r < n_short && nown < 64; r += 256) pos[nown++] = off[r];
+    for (int64_t e = 0; e < ne; ++e) {
+      const int32_t r = lrows[e];
+      if ((r - rbase) >= 0 && ((r - rbase) % 256) == 0) {
+        const int q = (int)((r - rbase) / 256);
+        if (q < nown) {
+          const int32_t p = pos[q]++;
+          cols[p] = (int32_t)(j0 + e / k);
+          cvals[p] = vals[j0 * k + e];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SJLT applications
+// ---------------------------------------------------------------------------
+
+// out[i, :] = sum_p vals[p] * A[cols[p], :], one 256-thread block per output row
+__global__ __launch_bounds__(256) void sjlt_left_kernel(int64_t d, int64_t n, const int64_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ cols,
+                                                        const double* __restrict__ vals,
+                                                        const double* __restrict__ A, int64_t lda,
+                                                        double* __restrict__ out, int64_t ldo) {
+  const int64_t i = blockIdx.x;
+  const int64_t p0 = rowptr[i], p1 = rowptr[i + 1];
+  for (int64_t c0 = (int64_t)blockIdx.y * 1024; c0 < min(n, (int64_t)(blockIdx.y + 1) * 1024); c0 += 256) {
+    const int64_t c = c0 + threadIdx.x;
+    if (c >= n) break;
+    double s = 0.0;
+    int64_t p = p0;
+    for (; p + 4 <= p1; p += 4) {
+      const double a0 = A[(int64_t)cols[p] * lda + c];
+      const double a1 = A[(int64_t)cols[p + 1] * lda + c];
+      const double a2 = A[(int64_t)cols[p + 2] * lda + c];
+      const double a3 = A[(int64_t)cols[p + 3] * lda + c];
+      s += vals[p] * a0;
+      s += vals[p + 1] * a1;
+      s += vals[p + 2] * a2;
+      s += vals[p + 3] * a3;
+    }
+    for (; p < p1; ++p) s += vals[p] * A[(int64_t)cols[p] * lda + c];
+    out[i * ldo + c] = s;
+  }
+}
+
+constexpr int RT_ROWS = 16;   // rows of A per block
+constexpr int RT_COLS = 512;  // columns of A per LDS tile
+
+// out[r, c] = sum_{p in row c of W} vals[p] * A[r, cols[p]]
+template <int Q>  // output columns per thread = Q (k <= 256 * Q)
+__global__ __launch_bounds__(256) void sjlt_right_kernel(int64_t m, int64_t n, int64_t k,
+                                                         const int64_t* __restrict__ rowptr,
+                                                         const int32_t* __restrict__ cols,
+                                                         const double* __restrict__ vals,
+                                                         const double* __restrict__ A, int64_t lda,
+                                                         double* __restrict__ out, int64_t ldo) {
+  __shared__ double As[RT_ROWS][RT_COLS];
+  const int64_t r0 = (int64_t)blockIdx.x * RT_ROWS;
+  const int nr = (int)imin64(RT_ROWS, m - r0);
+  double acc[Q][RT_ROWS];
+  int64_t ptr[Q], pend[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int64_t c = threadIdx.x + 256 * q;
+    ptr[q] = (c < k) ? rowptr[c] : 0;
+    pend[q] = (c < k) ? rowptr[c + 1] : 0;
+#pragma unroll
+    for (int rr = 0; rr < RT_ROWS; ++rr) acc[q][rr] = 0.0;
+  }
+  for (int64_t j0 = 0; j0 < n; j0 += RT_COLS) {
+    const int64_t jn = imin64(RT_COLS, n - j0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < RT_ROWS * RT_COLS; e += 256) {
+      const int rr = e / RT_COLS, jj = e % RT_COLS;
+      As[rr][jj] = (rr < nr && jj < jn) ? A[(r0 + rr) * lda + j0 + jj] : 0.0;
+    }
+    __syncthreads();
+    const int64_t j1 = j0 + jn;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      while (ptr[q] < pend[q]) {
+        const int32_t j = cols[ptr[q]];
+        if (j >= j1) break;
+        const double v = vals[ptr[q]];
+        const int jj = j - (int)j0;
+#pragma unroll
+        for (int rr = 0; rr < RT_ROWS; ++rr) acc[q][rr] += v * As[rr][jj];
+        ++ptr[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int64_t c = threadIdx.x + 256 * q;
+    if (c < k)
+#pragma unroll
+      for (int rr = 0; rr < RT_ROWS; ++rr)
+        if (rr < nr) out[(r0 + rr) * ldo + c] = acc[q][rr];
+  }
+}
+
+__global__ void transpose_kernel(int64_t rows, int64_t cols, const double* __restrict__ A, int64_t lda,
+                                 double* __restrict__ B, int64_t ldb) {
+  __shared__ double tile[32][33];
+  const int64_t bx = (int64_t)blockIdx.x * 32, by = (int64_t)blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t i = by + r, j = bx + tx;
+    if (i < rows && j < cols) tile[r][tx] = A[i * lda + j];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t j = bx + r, i = by + tx;  // B[j][i] = A[i][j]
+    if (i < rows && j < cols) B[j * ldb + i] = tile[tx][r];
+  }
+}
+
+}  // namespace
+
+extern "C" int pamd_gaussian_fill_f64(int64_t rows, int64_t cols, double scale, uint64_t seed, uint64_t offset,
+                                      double* X, int64_t ld, void* stream) {
+  PAMD_ARG_CHECK(rows >= 0, 1);
+  PAMD_ARG_CHECK(cols >= 0, 2);
+  PAMD_ARG_CHECK(X != nullptr || rows * cols == 0, 6);
+  PAMD_ARG_CHECK(ld >= cols && ld >= 1, 7);
+  const int64_t tot = rows * cols;
+  if (tot == 0) return PAMD_OK;
+  gaussian_fill_kernel<<<(unsigned)pamd_cdiv(tot, 256), 256, 0, (hipStream_t)stream>>>(rows, cols, scale, seed,
+                                                                                       offset, X, ld);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_sjlt_draw(int64_t n_short, int64_t n_long, int k, uint64_t seed, int32_t* rows, double* vals,
+                              void* stream) {
+  PAMD_ARG_CHECK(n_short >= 1 && n_short < (1LL << 31), 1);
+  PAMD_ARG_CHECK(n_long >= 0, 2);
+  PAMD_ARG_CHECK(k >= 1 && k <= 64, 3);
+  PAMD_ARG_CHECK(rows != nullptr || n_long == 0, 5);
+  PAMD_ARG_CHECK(vals != nullptr || n_long == 0, 6);
+  if (n_long == 0) return PAMD_OK;
+  sjlt_draw_kernel<<<(unsigned)pamd_cdiv(n_long, 256), 256, 0, (hipStream_t)stream>>>(n_short, n_long, k, seed,
+                                                                                      rows, vals);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+#define PAMD_CSR_MAX_SHORT 16384
+
+extern "C" int pamd_sjlt_csr_workspace(int64_t n_short, int64_t n_long, int64_t* bytes) {
+  PAMD_ARG_CHECK(n_short >= 1 && n_short <= PAMD_CSR_MAX_SHORT, 1);
+  PAMD_ARG_CHECK(n_long >= 0, 2);
+  PAMD_ARG_CHECK(bytes != nullptr, 3);
+  *bytes = imax64(1, pamd_cdiv(n_long, CSR_CB)) * n_short * (int64_t)sizeof(int32_t);
+  return PAMD_OK;
+}
+
+extern "C" int pamd_sjlt_csr(int64_t n_short, int64_t n_long, int k, const int32_t* rows, const double* vals,
+                             int64_t* rowptr, int32_t* cols, double* cvals, void* workspace, void* stream) {
+  PAMD_ARG_CHECK(n_short >= 1 && n_short <= PAMD_CSR_MAX_SHORT, 1);
+  PAMD_ARG_CHECK(n_long >= 0 && n_long * k < (1LL << 31), 2);
+  PAMD_ARG_CHECK(k >= 1 && k <= 64, 3);
+  PAMD_ARG_CHECK(rows != nullptr || n_long == 0, 4);
+  PAMD_ARG_CHECK(vals != nullptr || n_long == 0, 5);
+  PAMD_ARG_CHECK(rowptr != nullptr, 6);
+  PAMD_ARG_CHECK(cols != nullptr || n_long == 0, 7);
+  PAMD_ARG_CHECK(cvals != nullptr || n_long == 0, 8);
+  PAMD_ARG_CHECK(workspace != nullptr, 9);
+  hipStream_t st = (hipStream_t)stream;
+  const int nblk = (int)imax64(1, pamd_cdiv(n_long, CSR_CB));
+  int32_t* cnt = (int32_t*)workspace;
+  csr_count_kernel<<<nblk, 256, n_short * sizeof(int32_t), st>>>(n_short, n_long, k, rows, cnt);
+  PAMD_LAUNCH_CHECK();
+  csr_scan_kernel<<<1, 1024, 0, st>>>(n_short, nblk, cnt, rowptr);
+  PAMD_LAUNCH_CHECK();
+  if (n_long > 0) {
+    csr_fill_kernel<<<nblk, 256, CSR_CB * k * sizeof(int32_t), st>>>(n_short, n_long, k, rows, vals, cnt, cols,
+                                                                    cvals);
+    PAMD_LAUNCH_CHECK();
+  }
+  return PAMD_OK;
+}
+
+extern "C" int pamd_sjlt_left_f64(int64_t d, int64_t m, int64_t n, const int64_t* rowptr, const int32_t* cols,
+                                  const double* vals, const double* A, int64_t lda, double* out, int64_t ldo,
+                                  void* stream) {
+  PAMD_ARG_CHECK(d >= 0 && d < (1LL << 31), 1);
+  PAMD_ARG_CHECK(m >= 0, 2);
+  PAMD_ARG_CHECK(n >= 0, 3);
+  PAMD_ARG_CHECK(rowptr != nullptr, 4);
+  PAMD_ARG_CHECK(A != nullptr || m * n == 0, 7);
+  PAMD_ARG_CHECK(lda >= n && lda >= 1, 8);
+  PAMD_ARG_CHECK(out != nullptr || d * n == 0, 9);
+  PAMD_ARG_CHECK(ldo >= n && ldo >= 1, 10);
+  if (d == 0 || n == 0) return PAMD_OK;
+  dim3 grid((unsigned)d, (unsigned)pamd_cdiv(n, 1024));
+  sjlt_left_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(d, n, rowptr, cols, vals, A, lda, out, ldo);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_sjlt_right_f64(int64_t m, int64_t n, int64_t k, const int64_t* rowptr, const int32_t* cols,
+                                   const double* vals, const double* A, int64_t lda, double* out, int64_t ldo,
+                                   void* stream) {
+  PAMD_ARG_CHECK(m >= 0, 1);
+  PAMD_ARG_CHECK(n >= 0 && n < (1LL << 31), 2);
+  PAMD_ARG_CHECK(k >= 0 && k <= 1024, 3);
+  PAMD_ARG_CHECK(rowptr != nullptr, 4);
+  PAMD_ARG_CHECK(A != nullptr || m * n == 0, 7);
+  PAMD_ARG_CHECK(lda >= n && lda >= 1, 8);
+  PAMD_ARG_CHECK(out != nullptr || m * k == 0, 9);
+  PAMD_ARG_CHECK(ldo >= k && ldo >= 1, 10);
+  if (m == 0 || k == 0) return PAMD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = (unsigned)pamd_cdiv(m, RT_ROWS);
+  if (k <= 256)
+    sjlt_right_kernel<1><<<nb, 256, 0, st>>>(m, n, k, rowptr, cols, vals, A, lda, out, ldo);
+  else if (k <= 512)
+    sjlt_right_kernel<2><<<nb, 256, 0, st>>>(m, n, k, rowptr, cols, vals, A, lda, out, ldo);
+  else
+    sjlt_right_kernel<4><<<nb, 256, 0, st>>>(m, n, k, rowptr, cols, vals, A, lda, out, ldo);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_transpose_f64(int64_t rows, int64_t cols, const double* A, int64_t lda, double* B, int64_t ldb,
+                                  void* stream) {
+  PAMD_ARG_CHECK(rows >= 0, 1);
+  PAMD_ARG_CHECK(cols >= 0, 2);
+  PAMD_ARG_CHECK(A != nullptr || rows * cols == 0, 3);
+  PAMD_ARG_CHECK(lda >= cols && lda >= 1, 4);
+  PAMD_ARG_CHECK(B != nullptr || rows * cols == 0, 5);
+  PAMD_ARG_CHECK(ldb >= rows && ldb >= 1, 6);
+  if (rows == 0 || cols == 0) return PAMD_OK;
+  dim3 grid((unsigned)pamd_cdiv(cols, 32), (unsigned)pamd_cdiv(rows, 32));
+  transpose_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(rows, cols, A, lda, B, ldb);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}

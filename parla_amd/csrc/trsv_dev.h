@@ -1,0 +1,83 @@
+// Block-level upper-triangular solve used by the standalone TRSV entry point
+// and by the LSQR iteration kernel.
+//
+// The solution vector lives in LDS.  The triangle is processed in 64-row
+// blocks: the off-diagonal part of each block is a GEMV spread over all the
+// block's wavefronts (row-wise for R x = b, column-wise for R^T x = b, both
+// reading contiguous 512-byte row segments of the row-major R), then the
+// 64 x 64 diagonal block -- staged in LDS -- is solved by one wavefront with
+// one FMA per lane per step.
+#pragma once
+
+#include "common.h"
+
+#define PAMD_TRSV_MAX_N 2048
+constexpr int TRSV_SCRATCH_DOUBLES = 64 * 65 + 16 * 64;
+
+// All threads of the block (blockDim.x a multiple of 64, at most 1024) must
+// call this.  trans = 0 solves R x = b, trans = 1 solves R^T x = b; xs holds b
+// on entry and x on exit.  scratch: TRSV_SCRATCH_DOUBLES doubles of LDS.
+__device__ inline void trsv_upper_block(int trans, int n, const double* __restrict__ R, int64_t ldr,
+                                        double* xs, double* scratch) {
+  double(*Rd)[65] = reinterpret_cast<double(*)[65]>(scratch);
+  double(*red)[64] = reinterpret_cast<double(*)[64]>(scratch + 64 * 65);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int nb = (n + 63) / 64;
+  for (int bi = 0; bi < nb; ++bi) {
+    const int blk = trans ? bi : (nb - 1 - bi);
+    const int r0 = blk * 64;
+    const int bs = min(64, n - r0);
+    const int r1 = r0 + bs;
+    for (int e = tid; e < 64 * 64; e += blockDim.x) {
+      const int i = e >> 6, j = e & 63;
+      Rd[i][j] = (i < bs && j < bs) ? R[(int64_t)(r0 + i) * ldr + r0 + j] : 0.0;
+    }
+    if (!trans) {
+      // xs[r0:r1] -= R[r0:r1, r1:n] @ xs[r1:n]
+      if (r1 < n) {
+        for (int i = wave; i < bs; i += nwaves) {
+          const double* row = R + (int64_t)(r0 + i) * ldr;
+          double s = 0.0;
+          for (int j = r1 + lane; j < n; j += 64) s += row[j] * xs[j];
+          s = wave_sum(s);
+          if (lane == 0) xs[r0 + i] -= s;
+        }
+      }
+    } else {
+      // xs[r0:r1] -= R[0:r0, r0:r1]^T @ xs[0:r0]
+      double s = 0.0;
+      if (lane < bs)
+        for (int i = wave; i < r0; i += nwaves) s += R[(int64_t)i * ldr + r0 + lane] * xs[i];
+      red[wave][lane] = s;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      double y = (lane < bs) ? xs[r0 + lane] : 0.0;
+      if (trans) {
+        double t = 0.0;
+        for (int w = 0; w < nwaves; ++w) t += red[w][lane];
+        y -= t;
+        for (int i = 0; i < bs; ++i) {
+          const double xi = __shfl(y, i, 64) / Rd[i][i];
+          if (lane == i)
+            y = xi;
+          else if (lane > i)
+            y -= Rd[i][lane] * xi;
+        }
+      } else {
+        for (int i = bs - 1; i >= 0; --i) {
+          const double xi = __shfl(y, i, 64) / Rd[i][i];
+          if (lane == i)
+            y = xi;
+          else if (lane < i)
+            y -= Rd[lane][i] * xi;
+        }
+      }
+      if (lane < bs) xs[r0 + lane] = y;
+    }
+    __syncthreads();
+  }
+}

@@ -1,0 +1,350 @@
+"""Tensor-level wrappers over the C ABI (device memory and streams from torch).
+
+Every function here takes / returns float64 CUDA (HIP) tensors, enqueues work
+on torch's current stream and allocates its workspace through torch's caching
+allocator.  torch is used for memory and streams only; all arithmetic happens
+in libparla_amd.so.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+
+F64 = torch.float64
+
+# Optional per-launch timing: when a list is installed here, the wrappers
+# below bracket their library call with HIP events on the launch stream and
+# append (name, algorithmic flops, algorithmic bytes, start, end).
+_event_log = None
+
+
+def set_event_log(log):
+    """Install (list) or remove (None) the per-launch event log; returns the old one."""
+    global _event_log
+    old, _event_log = _event_log, log
+    return old
+
+
+class _Timed:
+    def __init__(self, name, flops=0.0, nbytes=0.0):
+        self.rec = None
+        if _event_log is not None:
+            self.rec = (name, float(flops), float(nbytes), torch.cuda.Event(enable_timing=True),
+                        torch.cuda.Event(enable_timing=True))
+
+    def __enter__(self):
+        if self.rec:
+            self.rec[3].record()
+        return self
+
+    def __exit__(self, *exc):
+        if self.rec:
+            self.rec[4].record()
+            _event_log.append(self.rec)
+        return False
+
+
+def lib():
+    return _lib.load()
+
+
+def default_device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("parla_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream(dev=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def as_device(X, dev=None):
+    """float64 CUDA tensor view of X (numpy array, CPU or CUDA tensor)."""
+    if isinstance(X, torch.Tensor):
+        if X.device.type == "cuda":
+            return X if X.dtype == F64 else X.to(F64)
+        X = X.detach().numpy()
+    arr = np.asarray(X, dtype=np.float64)
+    t = torch.from_numpy(np.ascontiguousarray(arr))
+    return t.to(dev or default_device())
+
+
+def like_input(t, ref):
+    """Return t in the kind of `ref`: CUDA tensor stays, anything else -> numpy."""
+    if isinstance(ref, torch.Tensor) and ref.device.type == "cuda":
+        return t
+    if isinstance(t, torch.Tensor):
+        return t.detach().cpu().numpy()
+    return t
+
+
+def workspace(nbytes, dev):
+    return torch.empty(max(int(nbytes), 8), dtype=torch.uint8, device=dev)
+
+
+def _query(fn, *args):
+    out = ctypes.c_int64(0)
+    check(fn(*args, ctypes.byref(out)), fn.__name__)
+    return out.value
+
+
+def mat_desc(X):
+    """(base, trans, ld): X == base (trans 0) or X == base^T (trans 1), base row-major with ld."""
+    if X.dim() != 2:
+        raise ValueError("expected a matrix")
+    r, c = X.shape
+    if r == 0 or c == 0:
+        return X.contiguous(), 0, max(c, 1)
+    s0, s1 = X.stride()
+    if s1 == 1 and (r == 1 or s0 >= c):
+        return X, 0, (s0 if r > 1 else c)
+    if s0 == 1 and (c == 1 or s1 >= r):
+        return X.t(), 1, (s1 if c > 1 else r)
+    return X.contiguous(), 0, c
+
+
+# ---------------------------------------------------------------------------
+# dense
+# ---------------------------------------------------------------------------
+
+def gemm(A, B, out=None, alpha=1.0, beta=0.0):
+    """out = alpha * A @ B + beta * out for 2-D float64 CUDA tensors (views allowed)."""
+    M, K = A.shape
+    K2, N = B.shape
+    if K != K2:
+        raise ValueError(f"matmul shape mismatch {tuple(A.shape)} @ {tuple(B.shape)}")
+    dev = A.device
+    if out is None:
+        out = torch.empty((M, N), dtype=F64, device=dev)
+        beta = 0.0
+    if out.stride(1) != 1 and N > 1:
+        raise ValueError("gemm output must have unit column stride")
+    At, ta, lda = mat_desc(A)
+    Bt, tb, ldb = mat_desc(B)
+    ldc = out.stride(0) if M > 1 else max(N, 1)
+    ws_bytes = _query(lib().pamd_gemm_workspace, M, N, K)
+    ws = workspace(ws_bytes, dev) if ws_bytes > 0 else None
+    with _Timed("gemm_f64", 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N)):
+        check(lib().pamd_gemm_f64(ta, tb, M, N, K, alpha, _p(At), lda, _p(Bt), ldb, beta, _p(out),
+                                  max(ldc, N, 1), _p(ws), ws_bytes, stream(dev)), "pamd_gemm_f64")
+    return out
+
+
+def gemv(A, x, trans=False, y=None, alpha=1.0, beta=0.0):
+    """y = alpha * op(A) x + beta * y, A row-major (unit column stride)."""
+    At, ta, lda = mat_desc(A)
+    if ta:  # A is a transposed view of At: op flips
+        trans = not trans
+    m, n = At.shape
+    dev = A.device
+    ylen = n if trans else m
+    if y is None:
+        y = torch.empty(ylen, dtype=F64, device=dev)
+        beta = 0.0
+    x = x.contiguous()
+    ws_bytes = _query(lib().pamd_gemv_workspace, int(trans), m, n)
+    ws = workspace(ws_bytes, dev) if ws_bytes > 0 else None
+    check(lib().pamd_gemv_f64(int(trans), m, n, alpha, _p(At), lda, _p(x), beta, _p(y), _p(ws),
+                              ws_bytes, stream(dev)), "pamd_gemv_f64")
+    return y
+
+
+def nrm2(x):
+    """||x||_2 as a 1-element device tensor."""
+    x = x.contiguous()
+    out = torch.empty(1, dtype=F64, device=x.device)
+    check(lib().pamd_nrm2_f64(x.numel(), _p(x), _p(out), stream(x.device)), "pamd_nrm2_f64")
+    return out
+
+
+def trsv_(R, x, trans=False):
+    """In place: x <- R^{-1} x (trans False) or R^{-T} x; R upper, unit column stride."""
+    if R.stride(1) != 1 and R.shape[1] > 1:
+        R = R.contiguous()
+    n = R.shape[0]
+    check(lib().pamd_trsv_upper_f64(int(trans), n, _p(R), max(R.stride(0), n), _p(x),
+                                    stream(x.device)), "pamd_trsv_upper_f64")
+    return x
+
+
+def transpose(A):
+    """Contiguous A^T."""
+    A = A.contiguous()
+    r, c = A.shape
+    out = torch.empty((c, r), dtype=F64, device=A.device)
+    check(lib().pamd_transpose_f64(r, c, _p(A), max(c, 1), _p(out), max(r, 1), stream(A.device)),
+          "pamd_transpose_f64")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# QR
+# ---------------------------------------------------------------------------
+
+class QRFactors:
+    """Result of geqrf: the factored matrix (R in its upper triangle) + reflectors."""
+
+    def __init__(self, M, n, ws):
+        self.M = M
+        self.m = M.shape[0]
+        self.n = n
+        self.ncols = M.shape[1]
+        self.ws = ws
+
+    @property
+    def R_view(self):
+        """n x n view whose upper triangle is R (strictly lower part holds reflectors)."""
+        return self.M[: self.n, : self.n]
+
+    def R(self):
+        return torch.triu(self.R_view)
+
+    def Q(self):
+        Q = torch.empty((self.m, self.n), dtype=F64, device=self.M.device)
+        check(lib().pamd_orgqr_f64(self.m, self.n, self.ncols, _p(self.ws), _p(Q), max(self.n, 1),
+                                   stream(Q.device)), "pamd_orgqr_f64")
+        return Q
+
+    def singular(self):
+        flag = ctypes.c_void_p(0)
+        check(lib().pamd_qr_singular_flag(_p(self.ws), self.m, self.n, self.ncols, ctypes.byref(flag)),
+              "pamd_qr_singular_flag")
+        base = self.ws.data_ptr()
+        off = flag.value - base
+        return bool(self.ws[off:off + 4].view(torch.int32).item())
+
+
+def geqrf_(M, n=None):
+    """Householder QR of the first n columns of M (m x ncols, contiguous), in place."""
+    if not M.is_contiguous():
+        raise ValueError("geqrf_ needs a contiguous matrix")
+    m, ncols = M.shape
+    n = ncols if n is None else n
+    if n > m:
+        raise ValueError("geqrf_ needs m >= n")
+    dev = M.device
+    ws = workspace(_query(lib().pamd_geqrf_workspace, m, n, ncols), dev)
+    check(lib().pamd_geqrf_f64(m, n, ncols, _p(M), max(ncols, 1), _p(ws), stream(dev)), "pamd_geqrf_f64")
+    return QRFactors(M, n, ws)
+
+
+def qr_q(Y):
+    """Economic Q of Y (m x n): la.qr(Y, mode='economic')[0], Q is m x min(m, n)."""
+    M = Y.clone(memory_format=torch.contiguous_format)
+    return geqrf_(M, min(M.shape)).Q()
+
+
+def lu_pl(A):
+    """P @ L of the partially pivoted LU of A (scipy.linalg.lu(A, permute_l=True)[0])."""
+    At = A if A.stride(-1) == 1 else A.contiguous()
+    m, n = At.shape
+    k = min(m, n)
+    dev = At.device
+    PL = torch.empty((m, k), dtype=F64, device=dev)
+    ws = workspace(_query(lib().pamd_lu_workspace, m, n), dev)
+    check(lib().pamd_lu_pl_f64(m, n, _p(At), max(At.stride(0), n, 1), _p(PL), max(k, 1), _p(ws), stream(dev)),
+          "pamd_lu_pl_f64")
+    return PL
+
+
+# ---------------------------------------------------------------------------
+# sketches
+# ---------------------------------------------------------------------------
+
+def gaussian_fill(rows, cols, scale, seed, dev, offset=0):
+    out = torch.empty((rows, cols), dtype=F64, device=dev)
+    check(lib().pamd_gaussian_fill_f64(rows, cols, scale, seed & (2 ** 64 - 1), offset, _p(out),
+                                       max(cols, 1), stream(dev)), "pamd_gaussian_fill_f64")
+    return out
+
+
+def sjlt_draw(n_short, n_long, k, seed, dev):
+    rows = torch.empty((n_long, k), dtype=torch.int32, device=dev)
+    vals = torch.empty((n_long, k), dtype=F64, device=dev)
+    check(lib().pamd_sjlt_draw(n_short, n_long, k, seed & (2 ** 64 - 1), _p(rows), _p(vals), stream(dev)),
+          "pamd_sjlt_draw")
+    return rows, vals
+
+
+def sjlt_csr(n_short, n_long, k, rows, vals):
+    """CSR (rowptr int64, cols int32, vals) of the wide SJLT whose CSC draw is (rows, vals)."""
+    dev = rows.device
+    rowptr = torch.empty(n_short + 1, dtype=torch.int64, device=dev)
+    cols = torch.empty(max(n_long * k, 1), dtype=torch.int32, device=dev)
+    cvals = torch.empty(max(n_long * k, 1), dtype=F64, device=dev)
+    ws = workspace(_query(lib().pamd_sjlt_csr_workspace, n_short, n_long), dev)
+    check(lib().pamd_sjlt_csr(n_short, n_long, k, _p(rows), _p(vals), _p(rowptr), _p(cols), _p(cvals),
+                              _p(ws), stream(dev)), "pamd_sjlt_csr")
+    return rowptr, cols, cvals
+
+
+def sjlt_left(rowptr, cols, vals, d, A, out=None):
+    """out (d x n) = W @ A with W (d x m) in CSR."""
+    A = A if A.stride(-1) == 1 else A.contiguous()
+    m, n = A.shape
+    if out is None:
+        out = torch.empty((d, n), dtype=F64, device=A.device)
+    check(lib().pamd_sjlt_left_f64(d, m, n, _p(rowptr), _p(cols), _p(vals), _p(A), max(A.stride(0), n, 1),
+                                   _p(out), max(out.stride(0), n, 1), stream(A.device)), "pamd_sjlt_left_f64")
+    return out
+
+
+def sjlt_right(rowptr, cols, vals, k, A, out=None):
+    """out (m x k) = A @ W^T with W (k x n) in CSR."""
+    A = A if A.stride(-1) == 1 else A.contiguous()
+    m, n = A.shape
+    if out is None:
+        out = torch.empty((m, k), dtype=F64, device=A.device)
+    check(lib().pamd_sjlt_right_f64(m, n, k, _p(rowptr), _p(cols), _p(vals), _p(A), max(A.stride(0), n, 1),
+                                    _p(out), max(out.stride(0), k, 1), stream(A.device)), "pamd_sjlt_right_f64")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# LSQR
+# ---------------------------------------------------------------------------
+
+def lsqr_run(A, b, R=None, sqrt_delta=0.0, x0=None, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8,
+             iter_lim=None, calc_var=False):
+    """Device LSQR on [A; sqrt_delta I] R^{-1} with rhs [b; 0].  Returns (x, var, arnorms, result)."""
+    A = A if A.stride(-1) == 1 else A.contiguous()
+    m, n = A.shape
+    dev = A.device
+    if iter_lim is None:
+        iter_lim = 2 * n
+    iter_lim = int(iter_lim)
+    b = b.contiguous()
+    prob = _lib.LsqrProblem()
+    prob.m, prob.n = m, n
+    prob.A, prob.lda = A.data_ptr(), max(A.stride(0), n)
+    prob.sqrt_delta = float(sqrt_delta)
+    if R is not None:
+        if R.stride(1) != 1:
+            R = R.contiguous()
+        prob.R, prob.ldr = R.data_ptr(), max(R.stride(0), n)
+    else:
+        prob.R, prob.ldr = None, n
+    prob.b = b.data_ptr()
+    if x0 is not None:
+        x0 = x0.contiguous()
+        prob.x0 = x0.data_ptr()
+    else:
+        prob.x0 = None
+    prob.damp, prob.atol, prob.btol, prob.conlim = float(damp), float(atol), float(btol), float(conlim)
+    prob.iter_lim = iter_lim
+    prob.calc_var = int(bool(calc_var))
+    x = torch.empty(n, dtype=F64, device=dev)
+    var = torch.zeros(n, dtype=F64, device=dev)
+    arn = torch.empty(max(iter_lim, 1), dtype=F64, device=dev)
+    ws = workspace(_query(lib().pamd_lsqr_workspace, m, n, iter_lim), dev)
+    res = _lib.LsqrResult()
+    check(lib().pamd_lsqr_run_f64(ctypes.byref(prob), _p(x), _p(var), _p(arn), ctypes.byref(res), _p(ws),
+                                  stream(dev)), "pamd_lsqr_run_f64")
+    return x, var, arn, res

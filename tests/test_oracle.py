@@ -303,3 +303,36 @@ def test_rs1_power_method_converges():
         gaps.append(la.norm(A.T @ (A @ vec) - vec))
     fit, r2 = orc.loglinear_fit(np.array([1, 4, 7, 10, 13]), np.array(gaps))
     assert r2 >= 0.7 and fit[1] <= -0.5
+
+
+def test_unpreconditioned_lsqr_trajectory_is_rounding_sensitive(golden):
+    """Documents why the GPU trajectory tests use a perturbation envelope: one
+    extra rounding in A^T y leaves the first 6 log entries at roundoff level
+    but moves entry 7 on, and shifts the iteration count -- while the
+    preconditioned run (kappa ~ 3) stays put."""
+    A, b = golden["lsqr_A60"], golden["lsqr_b60"]
+
+    class Rounded(orc.DenseOp):
+        def rmatvec(self, y):
+            return (self.A.T @ (3.0 * y)) / 3.0
+
+    ref = golden["lsqr_plain_arnorms"]
+    pert = orc.lsqr(Rounded(A), b, atol=1e-10, btol=1e-10)[7]
+    d = np.abs(pert[:ref.size] - ref[:pert.size]) / ref[:pert.size]
+    assert d[:5].max() < 1e-14
+    assert d[6:].max() > 1e-3
+    A_pc = orc.a_lift_precond(A, 0.0, golden["lsqr_R60"], True)[0]
+
+    class RoundedPc:
+        shape = A_pc.shape
+
+        def matvec(self, x):
+            return A_pc.matvec(x)
+
+        def rmatvec(self, y):
+            return A_pc.rmatvec(3.0 * y) / 3.0
+
+    refp = golden["lsqr_pc_arnorms"]
+    pertp = orc.lsqr(RoundedPc(), b, atol=1e-12, btol=1e-12, iter_lim=50)[7]
+    k = min(refp.size, pertp.size)
+    assert np.max(np.abs(pertp[:k] - refp[:k]) / refp[:k]) < 1e-6
