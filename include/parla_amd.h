@@ -177,6 +177,24 @@ int pamd_qr_singular_flag(const void* workspace, int64_t m, int64_t n,
 int pamd_transpose_f64(int64_t rows, int64_t cols, const double* A, int64_t lda,
                        double* B, int64_t ldb, void* stream);
 
+/* In-place Cholesky A = R^T R of the symmetric positive definite n x n A
+ * (upper triangle read and overwritten with R).  info[0] (device int) = 0, or
+ * j + 1 when the j-th pivot is not positive (factorisation stopped).  Used by
+ * CholeskyQR2 inside the QB stabilizer (utils/linalg_wrappers.py:6-7);
+ * n <= 512. */
+int pamd_potrf_upper_f64(int64_t n, double* A, int64_t lda, int* info,
+                         void* stream);
+
+/* Rinv (n x n, zeros below the diagonal) = R^{-1} for upper-triangular R
+ * (n <= 2048); one triangular solve per column, all columns in parallel. */
+int pamd_trtri_upper_f64(int64_t n, const double* R, int64_t ldr, double* Rinv,
+                         int64_t ldi, void* stream);
+
+/* In place B (m x n) <- B R^{-1} with R upper triangular n x n (n <= 2048):
+ * the Q = Y R^{-1} step of CholeskyQR. */
+int pamd_trsm_right_upper_f64(int64_t m, int64_t n, const double* R,
+                              int64_t ldr, double* B, int64_t ldb, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Preconditioned LSQR (device-resident recurrences)                         */
 /* ------------------------------------------------------------------------ */
@@ -206,14 +224,18 @@ int pamd_lsqr_workspace(int64_t m, int64_t n, int64_t iter_lim, int64_t* bytes);
 
 /* Run LSQR (comps/determiter/lsqr.py:98-574, the PARLA-modified Paige-Saunders
  * recurrences, same stopping tests and istop codes) on the operator
- * A_pc = [A; sqrt_delta I] R^{-1} (comps/preconditioning.py:16-67) with
- * right-hand side [b; 0], starting from x0.  Writes x (n, preconditioned
- * coordinates), var (n, if calc_var), arnorms (iter_lim; the per-iteration
- * ||A_pc^T r|| log of lsqr.py:413) and fills *result (host).  All recurrences
- * run on the device; this call blocks, reading the device state back once per
- * batch of iterations. */
-int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, double* var,
-                      double* arnorms, pamd_lsqr_result* result,
+ * A_pc = [A; sqrt_delta I] M, M = R^{-1} (comps/preconditioning.py:16-67),
+ * with right-hand side [b; 0], starting from x0 (preconditioned coordinates,
+ * the z0 of comps/determiter/saddle.py:197).  M is formed once, explicitly, by
+ * per-column triangular solves, and applied as a triangular GEMV inside the
+ * iteration.  Writes x (n, preconditioned coordinates), x_orig (n, optional:
+ * M x with that same M, i.e. the M_fwd(result[0]) of saddle.py:198), var (n,
+ * if calc_var), arnorms (iter_lim; the per-iteration ||A_pc^T r|| log of
+ * lsqr.py:413) and fills *result (host).  All recurrences run on the device;
+ * this call blocks, reading the device state back once per batch of
+ * iterations. */
+int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, double* x_orig,
+                      double* var, double* arnorms, pamd_lsqr_result* result,
                       void* workspace, void* stream);
 
 #ifdef __cplusplus

@@ -61,8 +61,11 @@ SIGNATURES = {
     "pamd_qr_singular_flag": (c_int, [c_vp, c_i64, c_i64, c_i64, ctypes.POINTER(c_vp)]),
     "pamd_lu_workspace": (c_int, [c_i64, c_i64, ctypes.POINTER(c_i64)]),
     "pamd_lu_pl_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
+    "pamd_potrf_upper_f64": (c_int, [c_i64, c_vp, c_i64, c_vp, c_vp]),
+    "pamd_trtri_upper_f64": (c_int, [c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
+    "pamd_trsm_right_upper_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
     "pamd_lsqr_workspace": (c_int, [c_i64, c_i64, c_i64, ctypes.POINTER(c_i64)]),
-    "pamd_lsqr_run_f64": (c_int, [ctypes.POINTER(LsqrProblem), c_vp, c_vp, c_vp,
+    "pamd_lsqr_run_f64": (c_int, [ctypes.POINTER(LsqrProblem), c_vp, c_vp, c_vp, c_vp,
                                   ctypes.POINTER(LsqrResult), c_vp, c_vp]),
 }
 

@@ -42,6 +42,14 @@ def _operands(A):
 
 def lsqr(A, b, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8, iter_lim=None, show=False,
          calc_var=False, x0=None):
+    return lsqr_with_solution(A, b, damp, atol, btol, conlim, iter_lim, show, calc_var, x0)[0]
+
+
+def lsqr_with_solution(A, b, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8, iter_lim=None, show=False,
+                       calc_var=False, x0=None):
+    """``(lsqr 10-tuple, x_orig)``: x_orig = M @ result[0] for a preconditioned
+    operator, computed with the explicit M the device iteration used (None for
+    a plain matrix or a non-triangular M)."""
     M, R, sd = _operands(A)
     m, n = M.shape
     bt = ops.as_device(b)
@@ -64,8 +72,9 @@ def lsqr(A, b, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8, iter_lim=None, show=F
         print('damp = %20.14e   calc_var = %8g' % (damp, calc_var))
         print('atol = %8.2e                 conlim = %8.2e' % (atol, conlim))
         print('btol = %8.2e               iter_lim = %8g' % (btol, iter_lim))
-    x, var, arn, res = ops.lsqr_run(M, bt, R=R, sqrt_delta=sd, x0=x0t, damp=damp, atol=atol,
-                                    btol=btol, conlim=conlim, iter_lim=iter_lim, calc_var=calc_var)
+    x, xo, var, arn, res = ops.lsqr_run(M, bt, R=R, sqrt_delta=sd, x0=x0t, damp=damp, atol=atol,
+                                        btol=btol, conlim=conlim, iter_lim=iter_lim, calc_var=calc_var,
+                                        want_orig=True)
     if res.n_log < 0:
         if show:
             print(_MSG[0])
@@ -83,5 +92,6 @@ def lsqr(A, b, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8, iter_lim=None, show=F
               % (res.itn, res.r2norm, res.acond, res.xnorm))
         print(' ')
     ref = b
-    return (ops.like_input(x, ref), int(res.istop), int(res.itn), res.r1norm, res.r2norm, res.anorm,
-            res.acond, arnorms, res.xnorm, ops.like_input(var, ref))
+    out = (ops.like_input(x, ref), int(res.istop), int(res.itn), res.r1norm, res.r2norm, res.anorm,
+           res.acond, arnorms, res.xnorm, ops.like_input(var, ref))
+    return out, (xo if R is not None else None)

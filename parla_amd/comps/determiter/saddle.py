@@ -3,7 +3,7 @@ branch of parla/comps/determiter/saddle.py:180-201)."""
 import torch
 
 from parla_amd import ops
-from parla_amd.comps.determiter.lsqr import lsqr
+from parla_amd.comps.determiter.lsqr import lsqr_with_solution
 from parla_amd.comps.preconditioning import a_lift_precond
 
 
@@ -33,8 +33,9 @@ class PcSS2(PrecondSaddleSolver):
             if delta > 0:
                 bt = torch.cat((bt, torch.zeros(n, dtype=bt.dtype, device=bt.device)))
             z0t = None if z0 is None else ops.as_device(z0)
-            result = lsqr(A_pc, bt, atol=tol, btol=tol, iter_lim=iter_lim, x0=z0t)
-            x = ops.as_device(M_fwd(result[0]))
+            result, x = lsqr_with_solution(A_pc, bt, atol=tol, btol=tol, iter_lim=iter_lim, x0=z0t)
+            if x is None:  # M applied by the library's triangular solve / GEMV
+                x = ops.as_device(M_fwd(ops.as_device(result[0])))
             y = None
             if want_y:
                 y = ops.gemv(At, x, y=bt[:m].clone(), alpha=-1.0, beta=1.0)

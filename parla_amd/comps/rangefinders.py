@@ -26,7 +26,8 @@ class RangeFinder:
 
 
 class RF1(RangeFinder):
-    """rangefinders.py:126-188: S = rso(A, k); Y = A @ S; Q = qr(Y)[0]."""
+    """rangefinders.py:126-188: S = rso(A, k); Y = A @ S; Q = qr(Y)[0] (Q up to column
+    signs, see parla_amd.utils.linalg_wrappers.orth)."""
 
     def __init__(self, rso: RowSketcher):
         self.rso = rso
@@ -41,5 +42,5 @@ class RF1(RangeFinder):
         At = ops.as_device(A)
         S = self.rso(At, k, rng)
         Y = usk.apply(At, S)
-        Q = ops.qr_q(Y)
+        Q = ops.orth_q(Y)
         return ops.like_input(Q, A)

@@ -15,6 +15,15 @@
 
 #define PAMD_WAVE 64
 
+// Order LDS accesses between lanes of one wavefront (no block barrier): a
+// compiler barrier plus wavefront-scope release/acquire fences.
+#define PAMD_WAVE_SYNC()                                   \
+  do {                                                     \
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); \
+    __builtin_amdgcn_wave_barrier();                       \
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
+  } while (0)
+
 // Return -idx from the calling entry point when cond is false.
 #define PAMD_ARG_CHECK(cond, idx) \
   do {                            \
@@ -39,18 +48,58 @@ __host__ __device__ static inline int64_t imax64(int64_t a, int64_t b) { return 
 
 // ---------------------------------------------------------------------------
 // wavefront / block reductions (64-wide wavefronts)
+//
+// Cross-lane traffic goes through DPP row operations (VALU, a few cycles) and
+// v_readlane, not ds_bpermute shuffles: on gfx950 a 64-lane FP64 shuffle tree
+// costs ~200 ns per reduction, the DPP sequence a small fraction of that.
 // ---------------------------------------------------------------------------
 
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+
+// value of v in lane `lane` (uniform index), for every lane
+__device__ __forceinline__ double wave_bcast(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ float wave_bcast(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Full-wave sum, returned to every lane.  quad_perm(1,0,3,2), quad_perm(2,3,0,1),
+// row_ror:4, row_ror:8 give each 16-lane row its total; row_bcast:15 (rows 1, 3)
+// and row_bcast:31 (rows 2, 3) fold the rows into lane 63.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, PAMD_WAVE);
-  return v;
+  v += dpp_f64<0xb1>(v);
+  v += dpp_f64<0x4e>(v);
+  v += dpp_f64<0x124>(v);
+  v += dpp_f64<0x128>(v);
+  v += dpp_f64<0x142, 0xa>(v);
+  v += dpp_f64<0x143, 0xc>(v);
+  return wave_bcast(v, 63);
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, PAMD_WAVE);
-  return v;
+  v += dpp_f32<0xb1>(v);
+  v += dpp_f32<0x4e>(v);
+  v += dpp_f32<0x124>(v);
+  v += dpp_f32<0x128>(v);
+  v += dpp_f32<0x142, 0xa>(v);
+  v += dpp_f32<0x143, 0xc>(v);
+  return wave_bcast(v, 63);
 }
 
 __device__ __forceinline__ double wave_max(double v) {

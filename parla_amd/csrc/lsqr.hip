@@ -1,27 +1,28 @@
 // Preconditioned LSQR with every recurrence on the device.
 //
-// Operator: A_pc = [A; sd I] R^{-1} (sd = sqrt(delta), R upper triangular, or
-// no R).  One LSQR iteration is three kernels:
+// Operator: A_pc = [A; sd I] M with M = R^{-1} (sd = sqrt(delta)), or M = I.
+// M is applied by backward-stable substitution (R x = v, R^T x = y) inside the
+// tail kernel, in LDS -- an explicit inverse would be faster to apply but its
+// forward error grows like kappa(R) u and costs an order of magnitude in the
+// final backward error.  One LSQR iteration is two kernels:
 //
-//   lsqr_pass    (multi-workgroup, one read of A): for each row i
-//                  u_i <- c1 * (A w)_i + c2 * u_i,   y += u_i * A[i, :]
-//                where w = R^{-1} v.  The bidiagonalisation needs A_pc v and
-//                then A_pc^T applied to the *normalised* new u; since
-//                A^T (u / beta) = (A^T u) / beta, both products come out of the
-//                same pass over A: HBM traffic is one read of A per iteration
-//                instead of two.
-//   lsqr_reduce  (one workgroup per 64 columns): y = sum of the per-workgroup
-//                partials in fixed order, plus the lifted rows sd * I.
-//   lsqr_step    (one 1024-thread workgroup): beta, v <- R^{-T} y / beta - beta v,
-//                alpha, the Givens recurrences and stopping tests of the PARLA
-//                lsqr (comps/determiter/lsqr.py:412-526), x / w updates, and
-//                w_next = R^{-1} v for the next pass.  It sets `done` when a
-//                stopping test fires; every kernel returns immediately once
-//                `done` is set, so the host can enqueue iterations in batches
-//                and read the state back once per batch.
+//   lsqr_pass  (multi-workgroup, one read of A): for each row i
+//                u_i <- c1 * (A w)_i + c2 * u_i,   y_g += u_i * A[i, :]
+//              where w = M v.  The bidiagonalisation needs A_pc v and then
+//              A_pc^T applied to the *normalised* new u; since
+//              A^T (u / beta) = (A^T u) / beta, both products come out of the
+//              same pass: one read of A per iteration instead of two.
+//   lsqr_tail  (one 1024-thread workgroup): sums the per-workgroup partials
+//              y_g in fixed order (plus the lifted rows sd * I), then beta,
+//              v <- R^{-T} y / beta - beta v, alpha, the Givens recurrences
+//              and stopping tests of PARLA's lsqr (comps/determiter/lsqr.py:
+//              412-526), the x / w updates and w_next = R^{-1} v.  The first call
+//              (phase 0) is the set-up of lsqr.py:362-395 instead.
 //
-// u is kept unnormalised: the 1/beta of lsqr.py:425 is folded into the next
-// pass's c2 = -alpha / beta.
+// Every kernel returns at once when `done` is set, so the host enqueues
+// iterations in batches and reads the state back once per batch.  u is kept
+// unnormalised: the 1/beta of lsqr.py:425 is folded into the next pass's
+// c2 = -alpha / beta.
 #include "common.h"
 #include "trsv_dev.h"
 
@@ -30,8 +31,8 @@ namespace {
 struct LsqrState {
   double alfa, beta, anorm, acond, ddnorm, res2, xnorm, xxnorm, z, cs2, sn2;
   double rhobar, phibar, rnorm, r1norm, r2norm, arnorm, bnorm;
-  double c1, c2, ssq_new, ssq_in;
-  int64_t itn, istop, done, early;
+  double c1, c2;
+  int64_t itn, istop, done, early, phase;
 };
 
 struct LsqrConst {
@@ -45,13 +46,12 @@ constexpr int PASS_THREADS = 256;
 __host__ __device__ inline int lsqr_grid(int64_t m) { return (int)imax64(1, imin64(1024, pamd_cdiv(m, 64))); }
 
 struct LsqrLayout {
-  int64_t st, u, v, w, work, y, pv, pb, pl, total;
+  int64_t st, u, v, w, work, y, pv, pb, total;
 };
 
 __host__ __device__ inline LsqrLayout lsqr_layout(int64_t m, int64_t n) {
   LsqrLayout L;
   const int G = lsqr_grid(m);
-  const int64_t nblk = pamd_cdiv(imax64(n, 1), 64);
   L.st = 0;
   L.u = 64;
   L.v = L.u + m + n;
@@ -60,8 +60,7 @@ __host__ __device__ inline LsqrLayout lsqr_layout(int64_t m, int64_t n) {
   L.y = L.work + n;
   L.pv = (L.y + n + 31) / 32 * 32;
   L.pb = L.pv + (int64_t)G * n;
-  L.pl = L.pb + (int64_t)G * 2;
-  L.total = L.pl + nblk * 2 + 32;
+  L.total = L.pb + (int64_t)G * 2 + 32;
   return L;
 }
 
@@ -69,7 +68,7 @@ __host__ __device__ inline LsqrLayout lsqr_layout(int64_t m, int64_t n) {
 // pass over A
 // ---------------------------------------------------------------------------
 // NPL = columns per lane held in registers (n <= 64 * NPL); NPL = 0: any n,
-// the row is read twice (second read hits cache).
+// the row is read twice (the second read hits cache).
 template <int NPL>
 __global__ __launch_bounds__(PASS_THREADS) void lsqr_pass_kernel(LsqrConst K, const double* __restrict__ A,
                                                                  double* __restrict__ ws) {
@@ -148,7 +147,6 @@ __global__ __launch_bounds__(PASS_THREADS) void lsqr_pass_kernel(LsqrConst K, co
       pv[(int64_t)blockIdx.x * n + c] = t;
     }
   } else {
-    // generic n: the workgroup owns rows; threads stride over columns
     for (int64_t c = threadIdx.x; c < n; c += PASS_THREADS) pv[(int64_t)blockIdx.x * n + c] = 0.0;
     __syncthreads();
     // uniform trip count across the block (the loop body has barriers)
@@ -191,44 +189,6 @@ __global__ __launch_bounds__(PASS_THREADS) void lsqr_pass_kernel(LsqrConst K, co
   }
 }
 
-// y[c] = sum_g pv[g][c] (+ lifted rows); block b also writes pl[b] partial sums
-__global__ __launch_bounds__(256) void lsqr_reduce_kernel(LsqrConst K, double* __restrict__ ws) {
-  LsqrState* S = reinterpret_cast<LsqrState*>(ws);
-  if (S->done) return;
-  const LsqrLayout L = lsqr_layout(K.m, K.n);
-  __shared__ double red[4][64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
-  const double* pv = ws + L.pv;
-  double s = 0.0;
-  if (c < K.n)
-    for (int g = wave; g < K.grid; g += 4) s += pv[(int64_t)g * K.n + c];
-  red[wave][lane] = s;
-  __syncthreads();
-  double lift_new = 0.0, lift_in = 0.0;
-  if (wave == 0 && c < K.n) {
-    double t = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
-    if (K.sd > 0.0) {
-      double* ul = ws + L.u + K.m;
-      const double uin = ul[c];
-      const double un = S->c1 * (K.sd * ws[L.work + c]) + S->c2 * uin;
-      ul[c] = un;
-      t += K.sd * un;
-      lift_new = un * un;
-      lift_in = uin * uin;
-    }
-    ws[L.y + c] = t;
-  }
-  if (wave == 0) {
-    lift_new = wave_sum(lift_new);
-    lift_in = wave_sum(lift_in);
-    if (lane == 0) {
-      ws[L.pl + 2 * blockIdx.x] = lift_new;
-      ws[L.pl + 2 * blockIdx.x + 1] = lift_in;
-    }
-  }
-}
-
 __device__ inline double dsign(double a) { return (a > 0.0) ? 1.0 : ((a < 0.0) ? -1.0 : 0.0); }
 
 // Stable Givens rotation of comps/determiter/lsqr.py:63-95.
@@ -254,30 +214,55 @@ __device__ inline void sym_ortho(double a, double b, double& c, double& s, doubl
   }
 }
 
-// Sum of the pass partials (fixed order).  All threads get the totals.
-__device__ inline void total_ssq(const LsqrConst& K, const double* ws, const LsqrLayout& L, double* red,
-                                 double& ssq_new, double& ssq_in) {
+// y = sum of pass partials (+ lifted rows) and the ssq totals.  Columns go
+// in chunks of 64 (lanes), the partial rows g are split over the 16 waves and
+// combined in wave order; every thread gets the totals, y is complete on return.
+__device__ inline void gather_partials(const LsqrConst& K, double* ws, const LsqrLayout& L, const LsqrState* S,
+                                       double* red, double (*cred)[64], double& ssq_new, double& ssq_in) {
+  const int64_t n = K.n;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const double* pv = ws + L.pv;
+  double* y = ws + L.y;
   double a = 0.0, b = 0.0;
   for (int g = threadIdx.x; g < K.grid; g += blockDim.x) {
     a += ws[L.pb + 2 * g];
     b += ws[L.pb + 2 * g + 1];
   }
-  const int nblk = (int)pamd_cdiv(imax64(K.n, 1), 64);
-  if (K.sd > 0.0)
-    for (int q = threadIdx.x; q < nblk; q += blockDim.x) {
-      a += ws[L.pl + 2 * q];
-      b += ws[L.pl + 2 * q + 1];
+  const double c1 = S->c1, c2 = S->c2;
+  for (int64_t c0 = 0; c0 < n; c0 += 64) {
+    const int64_t c = c0 + lane;
+    double t = 0.0;
+    if (c < n)
+      for (int g = wave; g < K.grid; g += nw) t += pv[(int64_t)g * n + c];
+    cred[wave][lane] = t;
+    __syncthreads();
+    if (wave == 0 && c < n) {
+      double tot = 0.0;
+      for (int wv = 0; wv < nw; ++wv) tot += cred[wv][lane];
+      if (K.sd > 0.0) {
+        double* ul = ws + L.u + K.m;
+        const double uin = ul[c];
+        const double un = c1 * (K.sd * ws[L.work + c]) + c2 * uin;
+        ul[c] = un;
+        tot += K.sd * un;
+        a += un * un;
+        b += uin * uin;
+      }
+      y[c] = tot;
     }
+    __syncthreads();
+  }
   ssq_new = block_sum(a, red);
   ssq_in = block_sum(b, red);
 }
 
-// v_out = R^{-T}(y * inv) + coef * v_in (LDS staging for the triangular solve).
-// Returns ||v_out||^2.  v_in may alias v_out.
-__device__ inline double apply_adjoint(const LsqrConst& K, const double* __restrict__ R, const double* y,
-                                       double inv, const double* v_in, double coef, double* v_out, double* xs,
-                                       double* scratch, double* red) {
+// v_out = R^{-T} (y * inv) + coef * v_in via substitution in LDS; returns
+// ||v_out||^2 (block-wide).  v_in may alias v_out.
+__device__ inline double apply_adjoint(const LsqrConst& K, const double* __restrict__ R, double* ws,
+                                       const LsqrLayout& L, double inv, const double* v_in, double coef,
+                                       double* v_out, double* xs, double* scratch, double* red) {
   const int64_t n = K.n;
+  const double* y = ws + L.y;
   double ss = 0.0;
   if (K.has_r) {
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) xs[i] = y[i] * inv;
@@ -290,8 +275,7 @@ __device__ inline double apply_adjoint(const LsqrConst& K, const double* __restr
     }
   } else {
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-      const double yi = y[i] * inv;
-      const double t = (coef != 0.0) ? yi + coef * v_in[i] : yi;
+      const double t = (coef != 0.0) ? y[i] * inv + coef * v_in[i] : y[i] * inv;
       v_out[i] = t;
       ss += t * t;
     }
@@ -299,23 +283,22 @@ __device__ inline double apply_adjoint(const LsqrConst& K, const double* __restr
   return block_sum(ss, red);
 }
 
-// work = R^{-1} v (or v)
-__device__ inline void apply_forward(const LsqrConst& K, const double* __restrict__ R, const double* v, double* work,
-                                     double* xs, double* scratch) {
-  const int64_t n = K.n;
+// out = R^{-1} v (or v); v complete before the call, out complete after
+__device__ inline void apply_forward(const LsqrConst& K, const double* __restrict__ R, const double* v,
+                                     double* out, double* xs, double* scratch) {
   if (K.has_r) {
+    for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) xs[i] = v[i];
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) xs[i] = v[i];
-    __syncthreads();
-    trsv_upper_block(0, (int)n, R, K.ldr, xs, scratch);
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) work[i] = xs[i];
+    trsv_upper_block(0, (int)K.n, R, K.ldr, xs, scratch);
+    for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) out[i] = xs[i];
   } else {
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) work[i] = v[i];
+    for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) out[i] = v[i];
   }
+  __syncthreads();
 }
 
-// Before the first pass: state, x = x0 or 0, u = b (lifted part 0),
-// work = R^{-1} x0 with (c1, c2) = (-1, 1), or work = 0 with (c1, c2) = (0, 1).
+// state, x = x0 or 0, u = b (lifted part 0), work = M x0 with (c1, c2) =
+// (-1, 1), or work = 0 with (c1, c2) = (0, 1)
 __global__ __launch_bounds__(1024) void lsqr_init_kernel(LsqrConst K, const double* __restrict__ R,
                                                          const double* __restrict__ b, const double* __restrict__ x0,
                                                          double* __restrict__ x, double* __restrict__ var,
@@ -330,8 +313,9 @@ __global__ __launch_bounds__(1024) void lsqr_init_kernel(LsqrConst K, const doub
     x[i] = K.has_x0 ? x0[i] : 0.0;
     if (var) var[i] = 0.0;
   }
+  __syncthreads();
   if (K.has_x0) {
-    apply_forward(K, R, x0, ws + L.work, smem, smem + K.n);
+    apply_forward(K, R, x, ws + L.work, smem, smem + K.n);
   } else {
     for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) ws[L.work + i] = 0.0;
   }
@@ -344,83 +328,78 @@ __global__ __launch_bounds__(1024) void lsqr_init_kernel(LsqrConst K, const doub
   }
 }
 
-// After the first pass (lsqr.py:362-395): beta, v, alfa, w and the early exit.
-__global__ __launch_bounds__(1024) void lsqr_setup_kernel(LsqrConst K, const double* __restrict__ R,
-                                                          double* __restrict__ x, double* __restrict__ ws) {
-  extern __shared__ double smem[];
-  __shared__ double red[16];
-  const LsqrLayout L = lsqr_layout(K.m, K.n);
-  LsqrState* S = reinterpret_cast<LsqrState*>(ws);
-  double* xs = smem;
-  double* scratch = smem + K.n;
-  double ssq_new, ssq_in;
-  total_ssq(K, ws, L, red, ssq_new, ssq_in);
-  const double bnorm = sqrt(ssq_in);
-  const double beta = sqrt(ssq_new);
-  double* v = ws + L.v;
-  double alfa = 0.0;
-  if (beta > 0.0) {
-    alfa = sqrt(apply_adjoint(K, R, ws + L.y, 1.0 / beta, v, 0.0, v, xs, scratch, red));
-  } else {
-    for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) v[i] = x[i];
-  }
-  if (alfa > 0.0) {
-    const double ia = 1.0 / alfa;
-    for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) v[i] = ia * v[i];
-  }
-  for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) ws[L.w + i] = v[i];
-  const double arnorm = alfa * beta;
-  const bool stop = (arnorm == 0.0) || (K.iter_lim <= 0);
-  if (!stop) apply_forward(K, R, v, ws + L.work, xs, scratch);
-  if (threadIdx.x == 0) {
-    S->bnorm = bnorm;
-    S->beta = beta;
-    S->alfa = alfa;
-    S->rhobar = alfa;
-    S->phibar = beta;
-    S->rnorm = S->r1norm = S->r2norm = beta;
-    S->arnorm = arnorm;
-    S->istop = 0;
-    S->itn = 0;
-    S->early = (arnorm == 0.0) ? 1 : 0;
-    S->done = stop ? 1 : 0;
-    S->c1 = 1.0;
-    S->c2 = -alfa * (1.0 / beta);
-  }
-}
-
-// One LSQR iteration after the pass (lsqr.py:413-526).
-__global__ __launch_bounds__(1024) void lsqr_step_kernel(LsqrConst K, const double* __restrict__ R,
+__global__ __launch_bounds__(1024) void lsqr_tail_kernel(LsqrConst K, const double* __restrict__ R,
                                                          double* __restrict__ x, double* __restrict__ var,
                                                          double* __restrict__ arnorms, double* __restrict__ ws) {
   LsqrState* S = reinterpret_cast<LsqrState*>(ws);
   if (S->done) return;
   extern __shared__ double smem[];
   __shared__ double red[16];
-  __shared__ double bc[8];
-  const LsqrLayout L = lsqr_layout(K.m, K.n);
+  __shared__ double cred[16][64];
+  __shared__ double bc[2];
   double* xs = smem;
   double* scratch = smem + K.n;
+  const LsqrLayout L = lsqr_layout(K.m, K.n);
   double* v = ws + L.v;
   double* w = ws + L.w;
   const int64_t n = K.n;
   const double eps = 2.220446049250313e-16;
 
   double ssq_new, ssq_in;
-  total_ssq(K, ws, L, red, ssq_new, ssq_in);
+  gather_partials(K, ws, L, S, red, cred, ssq_new, ssq_in);
   const double beta = sqrt(ssq_new);
+
+  if (S->phase == 0) {
+    // set-up after the first pass (lsqr.py:362-395)
+    const double bnorm = sqrt(ssq_in);
+    double alfa = 0.0;
+    if (beta > 0.0) {
+      alfa = sqrt(apply_adjoint(K, R, ws, L, 1.0 / beta, v, 0.0, v, xs, scratch, red));
+    } else {
+      for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = x[i];
+    }
+    __syncthreads();
+    if (alfa > 0.0) {
+      const double ia = 1.0 / alfa;
+      for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = ia * v[i];
+    }
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) w[i] = v[i];
+    __syncthreads();
+    const double arnorm = alfa * beta;
+    const bool stop = (arnorm == 0.0) || (K.iter_lim <= 0);
+    if (!stop) apply_forward(K, R, v, ws + L.work, xs, scratch);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      S->bnorm = bnorm;
+      S->beta = beta;
+      S->alfa = alfa;
+      S->rhobar = alfa;
+      S->phibar = beta;
+      S->rnorm = S->r1norm = S->r2norm = beta;
+      S->arnorm = arnorm;
+      S->istop = 0;
+      S->itn = 0;
+      S->early = (arnorm == 0.0) ? 1 : 0;
+      S->c1 = 1.0;
+      S->c2 = -alfa * (1.0 / beta);
+      S->phase = 1;
+      S->done = stop ? 1 : 0;
+    }
+    return;
+  }
+
   double alfa = S->alfa;
   double anorm = S->anorm;
   if (beta > 0.0) {
     anorm = sqrt(anorm * anorm + alfa * alfa + beta * beta + K.damp * K.damp);
-    const double a2 = apply_adjoint(K, R, ws + L.y, 1.0 / beta, v, -beta, v, xs, scratch, red);
-    alfa = sqrt(a2);
+    alfa = sqrt(apply_adjoint(K, R, ws, L, 1.0 / beta, v, -beta, v, xs, scratch, red));
+    __syncthreads();
     if (alfa > 0.0) {
       const double ia = 1.0 / alfa;
       for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = ia * v[i];
     }
   }
-  // scalar rotations (identical on every thread; no broadcast needed)
+  // scalar rotations (identical on every thread)
   const double damp = K.damp;
   const double rhobar0 = S->rhobar, phibar0 = S->phibar;
   const double rhobar1 = sqrt(rhobar0 * rhobar0 + damp * damp);
@@ -438,7 +417,7 @@ __global__ __launch_bounds__(1024) void lsqr_step_kernel(LsqrConst K, const doub
   const double t1 = phi / rho;
   const double t2 = -theta / rho;
   const double irho = 1.0 / rho;
-  __syncthreads();  // v complete before w = v + t2 w
+  __syncthreads();
   double dd = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
     const double wi = w[i];
@@ -510,10 +489,15 @@ __global__ __launch_bounds__(1024) void lsqr_step_kernel(LsqrConst K, const doub
     bc[0] = (istop != 0) ? 1.0 : 0.0;
   }
   __syncthreads();
-  if (bc[0] == 0.0) {
-    apply_forward(K, R, v, ws + L.work, xs, scratch);
-  }
+  if (bc[0] == 0.0) apply_forward(K, R, v, ws + L.work, xs, scratch);
   if (threadIdx.x == 0 && bc[0] != 0.0) S->done = 1;
+}
+
+// x_orig = R^{-1} x (the M_fwd of comps/determiter/saddle.py:198)
+__global__ __launch_bounds__(1024) void lsqr_xorig_kernel(LsqrConst K, const double* __restrict__ R,
+                                                          const double* __restrict__ x, double* __restrict__ xo) {
+  extern __shared__ double smem[];
+  apply_forward(K, R, x, xo, smem, smem + K.n);
 }
 
 int launch_pass(const LsqrConst& K, const double* A, double* ws, hipStream_t st) {
@@ -531,8 +515,6 @@ int launch_pass(const LsqrConst& K, const double* A, double* ws, hipStream_t st)
   else
     lsqr_pass_kernel<0><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
   PAMD_LAUNCH_CHECK();
-  lsqr_reduce_kernel<<<(unsigned)pamd_cdiv(n, 64), 256, 0, st>>>(K, ws);
-  PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }
 
@@ -547,25 +529,25 @@ extern "C" int pamd_lsqr_workspace(int64_t m, int64_t n, int64_t iter_lim, int64
   return PAMD_OK;
 }
 
-extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, double* var, double* arnorms,
-                                 pamd_lsqr_result* result, void* workspace, void* stream) {
+extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, double* x_orig, double* var,
+                                 double* arnorms, pamd_lsqr_result* result, void* workspace, void* stream) {
   PAMD_ARG_CHECK(prob != nullptr, 1);
   const pamd_lsqr_problem& P = *prob;
   PAMD_ARG_CHECK(P.m >= 0 && P.n >= 1, 1);
   PAMD_ARG_CHECK(P.A != nullptr && P.lda >= P.n, 1);
   PAMD_ARG_CHECK(P.b != nullptr, 1);
-  PAMD_ARG_CHECK(P.R == nullptr || (P.ldr >= P.n && P.n <= PAMD_TRSV_MAX_N), 1);
+  // with R the tail keeps x, the solve scratch and its own reduction space in LDS (<= 64 KiB)
+  PAMD_ARG_CHECK(P.R == nullptr || (P.ldr >= P.n && P.n <= 1900), 1);
   PAMD_ARG_CHECK(x != nullptr, 2);
-  PAMD_ARG_CHECK(var != nullptr || !P.calc_var, 3);
-  PAMD_ARG_CHECK(arnorms != nullptr || P.iter_lim <= 0, 4);
-  PAMD_ARG_CHECK(result != nullptr, 5);
-  PAMD_ARG_CHECK(workspace != nullptr, 6);
+  PAMD_ARG_CHECK(var != nullptr || !P.calc_var, 4);
+  PAMD_ARG_CHECK(arnorms != nullptr || P.iter_lim <= 0, 5);
+  PAMD_ARG_CHECK(result != nullptr, 6);
+  PAMD_ARG_CHECK(workspace != nullptr, 7);
   hipStream_t st = (hipStream_t)stream;
   LsqrConst K;
   K.m = P.m;
   K.n = P.n;
   K.lda = P.lda;
-  K.ldr = P.ldr;
   K.iter_lim = P.iter_lim;
   K.sd = P.sqrt_delta > 0 ? P.sqrt_delta : 0.0;
   K.damp = P.damp;
@@ -576,6 +558,7 @@ extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, doubl
   K.calc_var = P.calc_var != 0;
   K.has_x0 = P.x0 != nullptr;
   K.grid = lsqr_grid(P.m);
+  K.ldr = P.ldr;
   double* ws = (double*)workspace;
   const size_t shmem = (size_t)(K.has_r ? (P.n + TRSV_SCRATCH_DOUBLES) : 1) * sizeof(double);
 
@@ -583,7 +566,7 @@ extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, doubl
   PAMD_LAUNCH_CHECK();
   int rc = launch_pass(K, P.A, ws, st);
   if (rc) return rc;
-  lsqr_setup_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, ws);
+  lsqr_tail_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, var, arnorms, ws);  // phase 0: set-up
   PAMD_LAUNCH_CHECK();
 
   LsqrState hs;
@@ -597,11 +580,15 @@ extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, doubl
     for (int64_t it = 0; it < todo; ++it) {
       rc = launch_pass(K, P.A, ws, st);
       if (rc) return rc;
-      lsqr_step_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, var, arnorms, ws);
+      lsqr_tail_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, var, arnorms, ws);
       PAMD_LAUNCH_CHECK();
     }
     launched += todo;
     batch = imin64(batch * 2, 64);
+  }
+  if (x_orig) {
+    lsqr_xorig_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, x_orig);
+    PAMD_LAUNCH_CHECK();
   }
   result->istop = hs.istop;
   result->itn = hs.itn;

@@ -57,6 +57,7 @@ __global__ __launch_bounds__(1024) void qr_panel_kernel(int64_t m, int64_t c0, i
   __shared__ double red[16][QR_NB * (QR_NB + 1) / 2 + 1];
   __shared__ double taus[QR_NB];
   __shared__ double G[QR_NB][QR_NB];
+  __shared__ double Tl[QR_NB][QR_NB];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -171,7 +172,6 @@ __global__ __launch_bounds__(1024) void qr_panel_kernel(int64_t m, int64_t c0, i
         ++q;
       }
     // dlarft forward columnwise: T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j]
-    double Tl[QR_NB][QR_NB];
     for (int a = 0; a < QR_NB; ++a)
       for (int b = 0; b < QR_NB; ++b) Tl[a][b] = 0.0;
     for (int jj = 0; jj < nb; ++jj) {
@@ -186,6 +186,159 @@ __global__ __launch_bounds__(1024) void qr_panel_kernel(int64_t m, int64_t c0, i
     for (int a = 0; a < QR_NB; ++a) {
       tau_out[a] = (a < nb) ? taus[a] : 0.0;
       for (int b = 0; b < QR_NB; ++b) T[a * QR_NB + b] = Tl[a][b];
+    }
+  }
+}
+
+// Same factorisation with the panel held in registers: thread t owns the panel
+// rows t + 1024 q (q < RPT), so each column costs one fused reduction (sum of
+// squares + dot products, 8 values) and no memory traffic until the end.
+template <int RPT>
+__global__ __launch_bounds__(1024) void qr_panel_reg_kernel(int64_t m, int64_t c0, int nb, double* __restrict__ A,
+                                                            int64_t lda, double* __restrict__ V,
+                                                            double* __restrict__ T, double* __restrict__ tau_out,
+                                                            int* __restrict__ singular) {
+  __shared__ double red[16][QR_NB * (QR_NB - 1) / 2 + 1];
+  __shared__ double prow[QR_NB];
+  __shared__ double tots[QR_NB];
+  __shared__ double taus[QR_NB];
+  __shared__ double G[QR_NB][QR_NB];
+  __shared__ double Tl[QR_NB][QR_NB];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int64_t mr = m - c0;
+  double a[RPT][QR_NB];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t r = tid + 1024 * q;
+#pragma unroll
+    for (int c = 0; c < QR_NB; ++c) a[q][c] = (r < mr && c < nb) ? A[(c0 + r) * lda + c0 + c] : 0.0;
+  }
+#pragma unroll
+  for (int j = 0; j < QR_NB; ++j) {  // unrolled: every register index is static
+    if (j >= nb) break;
+    if (tid == j) {
+#pragma unroll
+      for (int c = 0; c < QR_NB; ++c) prow[c] = a[0][c];
+    }
+    double part[QR_NB];
+#pragma unroll
+    for (int c = 0; c < QR_NB; ++c) part[c] = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int64_t r = tid + 1024 * q;
+      if (r > j && r < mr) {
+        const double x = a[q][j];
+        part[0] += x * x;
+#pragma unroll
+        for (int c = 1; c < QR_NB; ++c)
+          if (c > j) part[c] += x * a[q][c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < QR_NB; ++c) {
+      const double sum = wave_sum(part[c]);
+      if (lane == 0) red[wave][c] = sum;
+    }
+    __syncthreads();
+    if (tid < QR_NB) {  // one lane per value sums the waves in order
+      double sum = 0.0;
+      for (int w = 0; w < nwaves; ++w) sum += red[w][tid];
+      tots[tid] = sum;
+    }
+    __syncthreads();
+    double tot[QR_NB];
+#pragma unroll
+    for (int c = 0; c < QR_NB; ++c) tot[c] = tots[c];
+    const double alpha = prow[j];
+    double beta, tau, scale;
+    if (tot[0] == 0.0) {
+      beta = alpha;
+      tau = 0.0;
+      scale = 0.0;
+    } else {
+      beta = -copysign(sqrt(alpha * alpha + tot[0]), alpha);
+      tau = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    double w[QR_NB];
+#pragma unroll
+    for (int c = 0; c < QR_NB; ++c) w[c] = (c > j && c < nb) ? tau * (prow[c] + scale * tot[c]) : 0.0;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int64_t r = tid + 1024 * q;
+      if (r > j && r < mr) {
+        const double v = a[q][j] * scale;
+        a[q][j] = v;
+#pragma unroll
+        for (int c = 1; c < QR_NB; ++c)
+          if (c > j) a[q][c] -= w[c] * v;
+      }
+    }
+    if (tid == j) {
+      a[0][j] = beta;
+#pragma unroll
+      for (int c = 1; c < QR_NB; ++c)
+        if (c > j) a[0][c] -= w[c];
+      taus[j] = tau;
+      if (beta == 0.0) singular[0] = 1;
+    }
+    __syncthreads();  // prow / red are rewritten by the next column
+  }
+  // write back: R above the diagonal, v below; V with explicit unit diagonal
+  constexpr int NPAIR = QR_NB * (QR_NB - 1) / 2;
+  double g[NPAIR];
+#pragma unroll
+  for (int p = 0; p < NPAIR; ++p) g[p] = 0.0;
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t r = tid + 1024 * q;
+    if (r < mr) {
+      double vr[QR_NB];
+#pragma unroll
+      for (int c = 0; c < QR_NB; ++c) {
+        if (c < nb) A[(c0 + r) * lda + c0 + c] = a[q][c];
+        vr[c] = (c >= nb || c > r) ? 0.0 : ((c == r) ? 1.0 : a[q][c]);
+        V[r * QR_NB + c] = vr[c];
+      }
+      int p = 0;
+#pragma unroll
+      for (int x = 0; x < QR_NB; ++x)
+#pragma unroll
+        for (int y = x + 1; y < QR_NB; ++y) g[p++] += vr[x] * vr[y];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NPAIR; ++p) {
+    const double sum = wave_sum(g[p]);
+    if (lane == 0) red[wave][p] = sum;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int p = 0;
+    for (int x = 0; x < QR_NB; ++x)
+      for (int y = x + 1; y < QR_NB; ++y) {
+        double sum = 0.0;
+        for (int w = 0; w < nwaves; ++w) sum += red[w][p];
+        G[x][y] = sum;
+        ++p;
+      }
+    for (int x = 0; x < QR_NB; ++x)
+      for (int y = 0; y < QR_NB; ++y) Tl[x][y] = 0.0;
+    for (int jj = 0; jj < nb; ++jj) {
+      const double tj = taus[jj];
+      Tl[jj][jj] = tj;
+      for (int i = 0; i < jj; ++i) {
+        double sum = 0.0;
+        for (int l = i; l < jj; ++l) sum += Tl[i][l] * G[l][jj];
+        Tl[i][jj] = -tj * sum;
+      }
+    }
+    for (int x = 0; x < QR_NB; ++x) {
+      tau_out[x] = (x < nb) ? taus[x] : 0.0;
+      for (int y = 0; y < QR_NB; ++y) T[x * QR_NB + y] = Tl[x][y];
     }
   }
 }
@@ -322,7 +475,16 @@ extern "C" int pamd_geqrf_f64(int64_t m, int64_t n, int64_t ncols, double* A, in
     const int nb = (int)imin64(QR_NB, n - c0);
     double* V = ws + L.v_off + qr_vrow(m, p) * QR_NB;
     double* T = ws + L.t_off + p * QR_NB * QR_NB;
-    qr_panel_kernel<<<1, 1024, 0, st>>>(m, c0, nb, A, lda, V, T, ws + L.tau_off + p * QR_NB, flag);
+    const int64_t mrows = m - c0;
+    double* tau = ws + L.tau_off + p * QR_NB;
+    if (mrows <= 1024)
+      qr_panel_reg_kernel<1><<<1, 1024, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
+    else if (mrows <= 2048)
+      qr_panel_reg_kernel<2><<<1, 1024, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
+    else if (mrows <= 4096)
+      qr_panel_reg_kernel<4><<<1, 1024, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
+    else
+      qr_panel_kernel<<<1, 1024, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
     PAMD_LAUNCH_CHECK();
     const int64_t mr = m - c0;
     const int64_t nc = ncols - (c0 + nb);

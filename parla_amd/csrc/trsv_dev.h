@@ -6,13 +6,14 @@
 // block's wavefronts (row-wise for R x = b, column-wise for R^T x = b, both
 // reading contiguous 512-byte row segments of the row-major R), then the
 // 64 x 64 diagonal block -- staged in LDS -- is solved by one wavefront with
-// one FMA per lane per step.
+// one FMA per lane per step (the pivot is broadcast with v_readlane and the
+// division replaced by a multiplication with the reciprocal of the diagonal).
 #pragma once
 
 #include "common.h"
 
 #define PAMD_TRSV_MAX_N 2048
-constexpr int TRSV_SCRATCH_DOUBLES = 64 * 65 + 16 * 64;
+constexpr int TRSV_SCRATCH_DOUBLES = 64 * 65 + 16 * 64 + 64;
 
 // All threads of the block (blockDim.x a multiple of 64, at most 1024) must
 // call this.  trans = 0 solves R x = b, trans = 1 solves R^T x = b; xs holds b
@@ -21,6 +22,7 @@ __device__ inline void trsv_upper_block(int trans, int n, const double* __restri
                                         double* xs, double* scratch) {
   double(*Rd)[65] = reinterpret_cast<double(*)[65]>(scratch);
   double(*red)[64] = reinterpret_cast<double(*)[64]>(scratch + 64 * 65);
+  double* rdiag = scratch + 64 * 65 + 16 * 64;  // 1 / R_ii of the current block
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -33,7 +35,9 @@ __device__ inline void trsv_upper_block(int trans, int n, const double* __restri
     const int r1 = r0 + bs;
     for (int e = tid; e < 64 * 64; e += blockDim.x) {
       const int i = e >> 6, j = e & 63;
-      Rd[i][j] = (i < bs && j < bs) ? R[(int64_t)(r0 + i) * ldr + r0 + j] : 0.0;
+      const double v = (i < bs && j < bs) ? R[(int64_t)(r0 + i) * ldr + r0 + j] : 0.0;
+      Rd[i][j] = v;
+      if (i == j) rdiag[i] = (i < bs) ? 1.0 / v : 0.0;
     }
     if (!trans) {
       // xs[r0:r1] -= R[r0:r1, r1:n] @ xs[r1:n]
@@ -56,24 +60,45 @@ __device__ inline void trsv_upper_block(int trans, int n, const double* __restri
     __syncthreads();
     if (wave == 0) {
       double y = (lane < bs) ? xs[r0 + lane] : 0.0;
+      const double rd = rdiag[lane];  // lane i holds 1 / R_ii
       if (trans) {
         double t = 0.0;
         for (int w = 0; w < nwaves; ++w) t += red[w][lane];
         y -= t;
-        for (int i = 0; i < bs; ++i) {
-          const double xi = __shfl(y, i, 64) / Rd[i][i];
-          if (lane == i)
-            y = xi;
-          else if (lane > i)
-            y -= Rd[i][lane] * xi;
+        // forward: lane l needs Rd[i][l] at step i; fetch 8 steps at a time
+        for (int i0 = 0; i0 < bs; i0 += 8) {
+          double r[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) r[k] = Rd[i0 + k][lane];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const int i = i0 + k;
+            if (i < bs) {
+              const double xi = wave_bcast(y, i) * wave_bcast(rd, i);
+              if (lane == i)
+                y = xi;
+              else if (lane > i)
+                y -= r[k] * xi;
+            }
+          }
         }
       } else {
-        for (int i = bs - 1; i >= 0; --i) {
-          const double xi = __shfl(y, i, 64) / Rd[i][i];
-          if (lane == i)
-            y = xi;
-          else if (lane < i)
-            y -= Rd[lane][i] * xi;
+        // backward: lane l needs Rd[l][i] at step i
+        for (int i0 = ((bs - 1) / 8) * 8; i0 >= 0; i0 -= 8) {
+          double r[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) r[k] = Rd[lane][i0 + k];
+#pragma unroll
+          for (int k = 7; k >= 0; --k) {
+            const int i = i0 + k;
+            if (i < bs) {
+              const double xi = wave_bcast(y, i) * wave_bcast(rd, i);
+              if (lane == i)
+                y = xi;
+              else if (lane < i)
+                y -= r[k] * xi;
+            }
+          }
         }
       }
       if (lane < bs) xs[r0 + lane] = y;

@@ -241,6 +241,69 @@ def qr_q(Y):
     return geqrf_(M, min(M.shape)).Q()
 
 
+CHOLQR_MAX_K = 512
+CHOLQR_MAX_COND = 1e7  # CholeskyQR2 is orthogonal to O(u) while kappa(Y) << u^{-1/2}
+
+
+def potrf_(G, info):
+    check(lib().pamd_potrf_upper_f64(G.shape[0], _p(G), max(G.stride(0), G.shape[0], 1), _p(info),
+                                     stream(G.device)), "pamd_potrf_upper_f64")
+
+
+def trtri(R):
+    """R^{-1} (upper triangular, zeros below) for upper-triangular R."""
+    n = R.shape[0]
+    Ri = torch.empty((n, n), dtype=F64, device=R.device)
+    check(lib().pamd_trtri_upper_f64(n, _p(R), max(R.stride(0), n, 1), _p(Ri), max(n, 1), stream(R.device)),
+          "pamd_trtri_upper_f64")
+    return Ri
+
+
+def trsm_right_(B, R):
+    """In place B <- B R^{-1} (R upper)."""
+    m, n = B.shape
+    check(lib().pamd_trsm_right_upper_f64(m, n, _p(R), max(R.stride(0), n, 1), _p(B), max(B.stride(0), n, 1),
+                                          stream(B.device)), "pamd_trsm_right_upper_f64")
+
+
+def cholqr2(Y):
+    """Q spanning range(Y) by CholeskyQR2 (two rounds of G = Q^T Q, R = chol(G),
+    Q <- Q R^{-1}); None when Y is too ill-conditioned for it (a pivot fails or
+    the first R's diagonal spread exceeds CHOLQR_MAX_COND)."""
+    m, k = Y.shape
+    if k == 0 or k > CHOLQR_MAX_K or m < k:
+        return None
+    Q = Y
+    info = torch.zeros(2, dtype=torch.int32, device=Y.device)
+    diag = None
+    for r in range(2):
+        G = gemm(Q.t(), Q)
+        potrf_(G, info[r:r + 1])
+        if r == 0:
+            diag = torch.diagonal(G).clone()
+        Q = gemm(Q, trtri(G))  # Y R^{-1} as an MFMA GEMM
+    # one synchronisation: pivots and the conditioning estimate
+    flags = info.cpu().numpy()
+    if flags.any():
+        return None
+    d = np.abs(diag.cpu().numpy())
+    if not np.all(np.isfinite(d)) or d.min() <= 0 or d.max() / d.min() > CHOLQR_MAX_COND:
+        return None
+    return Q
+
+
+def orth_q(Y):
+    """Orthonormal basis of range(Y) (m x min(m, n)).  CholeskyQR2 on MFMA when Y
+    is tall and well conditioned, else Householder; the two agree up to the sign
+    of each column."""
+    m, n = Y.shape
+    if m >= n:
+        Q = cholqr2(Y)
+        if Q is not None:
+            return Q
+    return qr_q(Y)
+
+
 def lu_pl(A):
     """P @ L of the partially pivoted LU of A (scipy.linalg.lu(A, permute_l=True)[0])."""
     At = A if A.stride(-1) == 1 else A.contiguous()
@@ -312,8 +375,12 @@ def sjlt_right(rowptr, cols, vals, k, A, out=None):
 # ---------------------------------------------------------------------------
 
 def lsqr_run(A, b, R=None, sqrt_delta=0.0, x0=None, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8,
-             iter_lim=None, calc_var=False):
-    """Device LSQR on [A; sqrt_delta I] R^{-1} with rhs [b; 0].  Returns (x, var, arnorms, result)."""
+             iter_lim=None, calc_var=False, want_orig=False):
+    """Device LSQR on [A; sqrt_delta I] R^{-1} with rhs [b; 0].
+
+    Returns (x, var, arnorms, result) -- or (x, x_orig, var, arnorms, result)
+    with ``want_orig``, x_orig = R^{-1} x computed with the iteration's own
+    explicit inverse."""
     A = A if A.stride(-1) == 1 else A.contiguous()
     m, n = A.shape
     dev = A.device
@@ -341,10 +408,13 @@ def lsqr_run(A, b, R=None, sqrt_delta=0.0, x0=None, damp=0.0, atol=1e-8, btol=1e
     prob.iter_lim = iter_lim
     prob.calc_var = int(bool(calc_var))
     x = torch.empty(n, dtype=F64, device=dev)
+    xo = torch.empty(n, dtype=F64, device=dev) if want_orig else None
     var = torch.zeros(n, dtype=F64, device=dev)
     arn = torch.empty(max(iter_lim, 1), dtype=F64, device=dev)
     ws = workspace(_query(lib().pamd_lsqr_workspace, m, n, iter_lim), dev)
     res = _lib.LsqrResult()
-    check(lib().pamd_lsqr_run_f64(ctypes.byref(prob), _p(x), _p(var), _p(arn), ctypes.byref(res), _p(ws),
-                                  stream(dev)), "pamd_lsqr_run_f64")
+    check(lib().pamd_lsqr_run_f64(ctypes.byref(prob), _p(x), _p(xo), _p(var), _p(arn), ctypes.byref(res),
+                                  _p(ws), stream(dev)), "pamd_lsqr_run_f64")
+    if want_orig:
+        return x, xo, var, arn, res
     return x, var, arn, res

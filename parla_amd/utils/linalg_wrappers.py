@@ -10,9 +10,15 @@ def _dev(S):
 
 
 def orth(S):
-    """utils/linalg_wrappers.py:6-7: Q factor of the economic Householder QR."""
+    """utils/linalg_wrappers.py:6-7: orthonormal basis of range(S).
+
+    Tall, well-conditioned S (the power-iteration case) goes through
+    CholeskyQR2 -- GEMM-shaped work on the MFMA units -- and otherwise through
+    Householder QR.  Q equals la.qr(S, mode='economic')[0] up to the sign of
+    each column, which no consumer (RS1, RF1, QB1, SVD1, EVD1) depends on.
+    """
     St, ref = _dev(S)
-    return ops.like_input(ops.qr_q(St), ref)
+    return ops.like_input(ops.orth_q(St), ref)
 
 
 def lu_stabilize(S):

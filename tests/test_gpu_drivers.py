@@ -22,31 +22,48 @@ def _say(msg):
 
 
 class _Rounded:
-    """Wraps an operator so A^T y is computed as (A^T (3 y)) / 3 and S @ X as
-    ((3 S) @ X) / 3: the same algebra with one extra rounding per product."""
+    """Wraps an operator so A^T y is computed as (A^T (c y)) / c and S @ X as
+    ((c S) @ X) / c: the same algebra with one extra rounding per product."""
 
-    def __init__(self, op):
+    def __init__(self, op, c=3.0):
         self.op = op
         self.shape = op.shape
+        self.c = c
 
     def matvec(self, x):
         return self.op @ x if not hasattr(self.op, "matvec") else self.op.matvec(x)
 
     def rmatvec(self, y):
         if hasattr(self.op, "rmatvec"):
-            return self.op.rmatvec(3.0 * y) / 3.0
-        return (self.op.T @ (3.0 * y)) / 3.0
+            return self.op.rmatvec(self.c * y) / self.c
+        return (self.op.T @ (self.c * y)) / self.c
 
     def __matmul__(self, X):
-        return ((3.0 * self.op) @ X) / 3.0
+        return ((self.c * self.op) @ X) / self.c
 
 
-def envelope(ref, perturbed, floor=1e-6, factor=1e3):
+ROUNDING_CONSTANTS = (3.0, 5.0, 7.0, 11.0, 13.0)
+
+
+def perturbed_change(ref, run):
+    """Largest relative change, per entry, that one extra rounding makes in
+    the oracle's sequence: `run(c)` reruns it with the constant c.  A single
+    perturbation is not enough -- past the point where LSQR's trajectory
+    turns chaotic its effect on one entry varies by three orders of
+    magnitude between constants."""
+    d = np.zeros(ref.size)
+    for c in ROUNDING_CONSTANTS:
+        p = np.atleast_1d(run(c))
+        k = min(p.size, ref.size)
+        d[:k] = np.maximum(d[:k], np.abs(p[:k] - ref[:k]) / np.abs(ref[:k]))
+        d[k:] = np.inf
+    return d
+
+
+def envelope(change, floor=1e-6, factor=1e3):
     """Per-entry tolerance: `factor` times the relative change one rounding
     perturbation makes in the oracle's own sequence (never below `floor`)."""
-    k = min(ref.size, perturbed.size)
-    d = np.abs(perturbed[:k] - ref[:k]) / np.abs(ref[:k])
-    return np.maximum(floor, factor * d)
+    return np.maximum(floor, factor * change)
 
 
 def assert_within_envelope(got, ref, env):
@@ -108,8 +125,8 @@ def test_lsqr_matches_reference_trajectory(pa, golden, tag, kw):
     assert res[1] == gi[0]
     assert abs(res[2] - gi[1]) <= 2
     garn = golden[f"lsqr_{tag}_arnorms"]
-    pert = orc.lsqr(_Rounded(orc.DenseOp(A)), b, **kw)
-    assert_within_envelope(np.atleast_1d(res[7]), garn, envelope(garn, np.atleast_1d(pert[7])))
+    change = perturbed_change(garn, lambda c: orc.lsqr(_Rounded(orc.DenseOp(A), c), b, **kw)[7])
+    assert_within_envelope(np.atleast_1d(res[7]), garn, envelope(change))
     np.testing.assert_allclose(res[0], golden[f"lsqr_{tag}_x"], rtol=1e-6, atol=1e-9)
     r1, r2, an, ac, xn = golden[f"lsqr_{tag}_scal"]
     np.testing.assert_allclose([res[3], res[4], res[8]], [r1, r2, xn], rtol=1e-6)
@@ -171,13 +188,15 @@ def test_spo_matches_reference_golden_runs(pa, golden):
         ref = golden[f"spo_{tag}_errors"]
         assert abs(log.errors.size - ref.size) <= 2, (tag, log.errors.size, ref.size)
 
-        def pgen(r, c, rng, kind=kind):
-            S = orc.sjlt_operator(r, c, rng, 8) if kind == "sjlt" else orc.gaussian_operator(r, c, rng)
-            return _Rounded(S)
+        def rerun(const, kind=kind, sf=sf, delta=delta):
+            def pgen(r, c, rng):
+                S = orc.sjlt_operator(r, c, rng, 8) if kind == "sjlt" else orc.gaussian_operator(r, c, rng)
+                return _Rounded(S, const)
 
-        _, plog = orc.spo(A, b, float(delta), 1e-12, 60, np.random.default_rng(5), sketch=pgen,
-                          sampling_factor=float(sf))
-        assert_within_envelope(log.errors, ref, envelope(ref, plog.errors, floor=1e-5))
+            return orc.spo(A, b, float(delta), 1e-12, 60, np.random.default_rng(5), sketch=pgen,
+                           sampling_factor=float(sf))[1].errors
+
+        assert_within_envelope(log.errors, ref, envelope(perturbed_change(ref, rerun), floor=1e-5))
 
 
 def test_spo_closed_form_reference_problems(pa):

@@ -263,3 +263,69 @@ def test_device_sjlt_draw_invariants(ops):
     assert np.all(np.abs(counts - expect) < 6 * np.sqrt(expect))
     rows2, _ = ops.sjlt_draw(50, 20000, 8, 99, d)
     np.testing.assert_array_equal(host(rows2), R)  # deterministic
+
+
+# --------------------------------------------------------------------------
+# CholeskyQR2 pieces
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 266, 512])
+def test_potrf_matches_scipy(ops, n):
+    _say(f"potrf n={n}")
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n + 5, n))
+    G = X.T @ X
+    Gd = dev(G.copy())
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ops.potrf_(Gd, info)
+    assert int(host(info)[0]) == 0
+    R = np.triu(host(Gd))
+    Rr = la.cholesky(G, lower=False)
+    assert la.norm(R - Rr) <= 1e3 * EPS * la.norm(Rr) * n
+
+
+def test_potrf_reports_failed_pivot(ops):
+    G = np.eye(40)
+    G[17, 17] = -1.0
+    Gd = dev(G)
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ops.potrf_(Gd, info)
+    assert int(host(info)[0]) == 18
+
+
+@pytest.mark.parametrize("m,n", [(100, 7), (8192, 266), (3000, 300), (500, 64)])
+def test_trsm_right_and_cholqr2(ops, m, n):
+    _say(f"trsm/cholqr2 m={m} n={n}")
+    rng = np.random.default_rng(m + n)
+    Y = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, -4, n))
+    R = la.qr(Y, mode="economic")[1]
+    B = dev(Y.copy())
+    ops.trsm_right_(B, dev(R))
+    ref = la.solve_triangular(R, Y.T, trans="T").T
+    assert rel(host(B), ref) < 1e-9
+    Q = ops.cholqr2(dev(Y))
+    assert Q is not None
+    Qh = host(Q)
+    assert la.norm(Qh.T @ Qh - np.eye(n)) <= 1e3 * EPS * np.sqrt(m)
+    # same range as Y: Q Q^T Y = Y
+    assert la.norm(Qh @ (Qh.T @ Y) - Y) <= 1e-12 * la.norm(Y)
+    # agrees with LAPACK's Q up to column signs
+    Ql = la.qr(Y, mode="economic")[0]
+    assert la.norm(np.abs(Ql.T @ Qh) - np.eye(n)) <= 1e-8
+
+
+def test_cholqr2_declines_rank_deficient(ops):
+    rng = np.random.default_rng(0)
+    Y = rng.standard_normal((300, 10)) @ rng.standard_normal((10, 20))  # rank 10
+    assert ops.cholqr2(dev(Y)) is None
+    Q = host(ops.orth_q(dev(Y)))  # falls back to Householder
+    assert la.norm(Q.T @ Q - np.eye(20)) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 266, 1024])
+def test_trtri_matches_scipy(ops, n):
+    rng = np.random.default_rng(n)
+    R = np.triu(rng.standard_normal((n, n))) + np.diag(4 + rng.random(n))
+    Ri = host(ops.trtri(dev(R)))
+    assert np.all(np.tril(Ri, -1) == 0)
+    assert la.norm(R @ Ri - np.eye(n)) <= 100 * n * EPS * la.norm(R) * la.norm(Ri)
