@@ -226,10 +226,11 @@ int pamd_lsqr_workspace(int64_t m, int64_t n, int64_t iter_lim, int64_t* bytes);
  * recurrences, same stopping tests and istop codes) on the operator
  * A_pc = [A; sqrt_delta I] M, M = R^{-1} (comps/preconditioning.py:16-67),
  * with right-hand side [b; 0], starting from x0 (preconditioned coordinates,
- * the z0 of comps/determiter/saddle.py:197).  M is formed once, explicitly, by
- * per-column triangular solves, and applied as a triangular GEMV inside the
- * iteration.  Writes x (n, preconditioned coordinates), x_orig (n, optional:
- * M x with that same M, i.e. the M_fwd(result[0]) of saddle.py:198), var (n,
+ * the z0 of comps/determiter/saddle.py:197).  M is applied by triangular
+ * substitution with R (one R x = y and one R^T x = y solve per iteration, as
+ * la.solve_triangular does at preconditioning.py:28,37), never as an explicit
+ * inverse.  Writes x (n, preconditioned coordinates), x_orig (n, optional:
+ * R^{-1} x, i.e. the M_fwd(result[0]) of saddle.py:198), var (n,
  * if calc_var), arnorms (iter_lim; the per-iteration ||A_pc^T r|| log of
  * lsqr.py:413) and fills *result (host).  All recurrences run on the device;
  * this call blocks, reading the device state back once per batch of

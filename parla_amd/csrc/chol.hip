@@ -7,7 +7,7 @@
 // iteration stabilizer (utils/linalg_wrappers.py:6-7) where Y is well enough
 // conditioned (the caller falls back to Householder when potrf fails).
 //
-// potrf: one 1024-thread workgroup, blocked right-looking with 32-wide
+// potrf: one 512-thread workgroup, blocked right-looking with 32-wide
 //   panels (see the kernel).  trtri: R^{-1} one column per workgroup, so that
 //   Y R^{-1} is a plain MFMA GEMM.  trsm_right: the in-place alternative for
 //   callers that cannot afford the k x k inverse; one workgroup per tile of
@@ -19,160 +19,153 @@ namespace {
 
 constexpr int CB = 32;
 constexpr int POTRF_MAX_N = 512;  // block row of R staged in LDS: 32 x 512 doubles
+constexpr int POTRF_THREADS = 512;
 
-// Blocked right-looking upper Cholesky, one 512-thread workgroup, 32-wide
-// blocks.  Per block:
-//   1. one wavefront factors the 32 x 32 diagonal block with its columns in
-//      registers (lane c holds column c; pivots and row entries move by
-//      v_readlane) and forms the block's inverse the same way;
-//   2. the block row R[jb:jb+32, c0:n] = D^{-T} G[jb:jb+32, c0:n] is a product
-//      with that inverse, spread over all threads (no sequential substitution);
-//   3. the trailing upper triangle gets the rank-32 update from LDS in 4 x 4
-//      register tiles.
-// The diagonal-block inverse has forward error ~ kappa(D) u; CholeskyQR2's
-// second round (kappa ~ 1) removes its effect on the final Q.
-__global__ __launch_bounds__(512) void potrf_kernel(int64_t n, double* __restrict__ A, int64_t lda,
-                                                    int* __restrict__ info) {
-  __shared__ double D[CB][CB + 1];
-  __shared__ double Di[CB][CB + 1];
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Blocked right-looking upper Cholesky in one 512-thread workgroup, 32-wide
+// blocks.  Per block (G = the not yet factored part of A):
+//   1. wavefront 0 factors the 32 x 32 diagonal block D = R11^T R11: lane c
+//      holds column c in registers; at step j the finished row j of R11 goes
+//      to LDS and every lane reads it back with same-address (broadcast)
+//      reads, so the only cross-lane move per step is the pivot's readlane;
+//   2. the block row R12 = R11^{-T} G12 is a forward substitution, one thread
+//      per column of G12, in axpy order (row l, once known, updates all later
+//      rows), with R11 broadcast from LDS and multiplications by 1 / R_ll;
+//   3. the trailing upper triangle G22 -= R12^T R12 is a rank-32 SYRK on
+//      v_mfma_f64_16x16x4f64, 16 x 16 upper-triangle tiles spread over the
+//      wavefronts.
+// Where a step needs a row of R11 or an operand tile, all of its LDS reads
+// are issued before the arithmetic (sched_barrier), so the step waits for
+// LDS once instead of once per read.
+// Rows nb..31 of the staged block are zero and the diagonal block is padded
+// with the identity, so a last block narrower than 32 needs no special case.
+__global__ __launch_bounds__(POTRF_THREADS) void potrf_kernel(int64_t n, double* __restrict__ A, int64_t lda,
+                                                              int* __restrict__ info) {
+  __shared__ double Rd[CB][CB + 1];  // the diagonal block; R11 row j replaces its row j at step j
+  __shared__ double rinv[CB];        // 1 / R_jj
   __shared__ double Rs[CB][POTRF_MAX_N];
   __shared__ int bad;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nthr = blockDim.x;
+  constexpr int nthr = POTRF_THREADS, nwaves = POTRF_THREADS / 64;
+  const int lr = lane & 15, lk = lane >> 4;  // MFMA operand coordinates
   if (tid == 0) bad = 0;
   __syncthreads();
   for (int64_t jb = 0; jb < n; jb += CB) {
     const int nb = (int)imin64(CB, n - jb);
     const int64_t c0 = jb + nb;
     const int nt = (int)(n - c0);
+    const int ntp = (nt + 15) & ~15;  // staged width, padded to whole tiles
     for (int e = tid; e < CB * CB; e += nthr) {
       const int i = e / CB, j = e % CB;
-      D[i][j] = (i < nb && j < nb) ? A[(jb + i) * lda + jb + j] : ((i == j) ? 1.0 : 0.0);
+      Rd[i][j] = (i < nb && j < nb) ? A[(jb + i) * lda + jb + j] : ((i == j) ? 1.0 : 0.0);
     }
-    for (int e = tid; e < nb * nt; e += nthr) {
-      const int i = e / nt, c = e % nt;
-      Rs[i][c] = A[(jb + i) * lda + c0 + c];
+    for (int e = tid; e < CB * ntp; e += nthr) {
+      const int i = e / ntp, c = e % ntp;
+      Rs[i][c] = (i < nb && c < nt) ? A[(jb + i) * lda + c0 + c] : 0.0;
     }
     __syncthreads();
     if (wave == 0) {
-      const int c = lane & (CB - 1);
+      const int c = pamd_opaque(lane & (CB - 1));
       double col[CB];
 #pragma unroll
-      for (int i = 0; i < CB; ++i) col[i] = D[i][c];
+      for (int i = 0; i < CB; ++i) col[i] = Rd[i][c];
+      PAMD_WAVE_SYNC();  // Rd is overwritten row by row below
       int fail = 0;
 #pragma unroll
       for (int j = 0; j < CB; ++j) {
         const double djj = wave_bcast(col[j], j);
         if (!fail && !(djj > 0.0 && djj < INFINITY)) fail = j + 1;
         if (fail) continue;  // wave-uniform: djj is a broadcast
-        const double r = sqrt(djj);
-        if (c == j)
-          col[j] = r;
-        else if (c > j)
-          col[j] = col[j] / r;
-        const double rjc = col[j];
+        // 1/sqrt(djj): hardware estimate + two Newton steps y <- y (3 - d y^2) / 2
+        // (each squares the relative error), then sqrt(djj) = djj * y
+        double y = __builtin_amdgcn_rsq(djj);
+        const double h = 0.5 * djj;
+        y = y * fma(-h * y, y, 1.5);
+        y = y * fma(-h * y, y, 1.5);
+        const double r = djj * y;
+        // R11[j][c] for c >= j; lanes c < j produce entries below the diagonal
+        // that no later step reads (lanes 32..63 repeat lanes 0..31)
+        const double v = (c == j) ? r : col[j] * y;
+        Rd[j][c] = v;
+        if (lane == 0) rinv[j] = y;
+        PAMD_WAVE_SYNC();
+        double rowj[CB];
 #pragma unroll
-        for (int i = j + 1; i < CB; ++i) {
-          const double rji = wave_bcast(col[j], i);
-          if (c >= i) col[i] -= rji * rjc;
-        }
+        for (int i = 0; i < CB; ++i) rowj[i] = (i > j) ? Rd[j][i] : 0.0;
+        __builtin_amdgcn_sched_barrier(0);  // issue the row's reads together, then wait once
+#pragma unroll
+        for (int i = 0; i < CB; ++i)  // constant trip count: unrolls before the j loop does
+          if (i > j) col[i] -= rowj[i] * v;
       }
-      if (fail) {
-        if (lane == 0 && fail <= nb) bad = (int)(jb + fail);
-      } else {
-        // D^{-1}, column c: backward substitution on e_c; D[i][l] is lane l's col[i]
-        double x[CB];
-#pragma unroll
-        for (int i = CB - 1; i >= 0; --i) {
-          const double rii = 1.0 / wave_bcast(col[i], i);
-          double sacc = 0.0;
-#pragma unroll
-          for (int l = i + 1; l < CB; ++l) {
-            const double dil = wave_bcast(col[i], l);
-            if (l <= c) sacc += dil * x[l];
-          }
-          x[i] = (i == c) ? rii : ((i < c) ? -sacc * rii : 0.0);
-        }
-        if (lane < CB) {
-#pragma unroll
-          for (int i = 0; i < CB; ++i) {
-            D[i][c] = (i <= c) ? col[i] : 0.0;
-            Di[i][c] = x[i];
-          }
-        }
-      }
+      if (fail && lane == 0 && fail <= nb) bad = (int)(jb + fail);
     }
     __syncthreads();
     if (bad) break;
     for (int e = tid; e < nb * nb; e += nthr) {
       const int i = e / nb, j = e % nb;
-      if (j >= i) A[(jb + i) * lda + jb + j] = D[i][j];
+      if (j >= i) A[(jb + i) * lda + jb + j] = Rd[i][j];
     }
-    // block row R[i][c] = sum_{l <= i} Di[l][i] G[l][c].  Row i needs G rows
-    // 0..i only, so rows are produced bottom-up in groups of 8 and written over
-    // the G rows they no longer need.
-    for (int ig = ((nb - 1) / 8) * 8; ig >= 0; ig -= 8) {
-      const int nrow = imin64(8, nb - ig);
-      const int nout = nrow * nt;
-      constexpr int MAXO = (8 * POTRF_MAX_N + 511) / 512;
-      double o[MAXO];
+    if (nt == 0) break;
+    // 2. block row, thread per column (nt < POTRF_THREADS, a single pass)
+    static_assert(POTRF_MAX_N - CB <= POTRF_THREADS, "one thread per trailing column");
+    if (tid < nt) {
+      const int c = tid;
+      double t[CB];
 #pragma unroll
-      for (int q = 0; q < MAXO; ++q) {
-        const int e = tid + q * nthr;
-        double acc = 0.0;
-        if (e < nout) {
-          const int i = ig + e / nt, cc = e % nt;
-          for (int l = 0; l <= i; ++l) acc += Di[l][i] * Rs[l][cc];
-        }
-        o[q] = acc;
-      }
-      __syncthreads();
+      for (int l = 0; l < CB; ++l) t[l] = Rs[l][c];
 #pragma unroll
-      for (int q = 0; q < MAXO; ++q) {
-        const int e = tid + q * nthr;
-        if (e < nout) {
-          const int i = ig + e / nt, cc = e % nt;
-          Rs[i][cc] = o[q];
-          A[(jb + i) * lda + c0 + cc] = o[q];
-        }
+      for (int l = 0; l < CB; ++l) {
+        double rowl[CB];
+#pragma unroll
+        for (int i = 0; i < CB; ++i) rowl[i] = (i > l) ? Rd[l][i] : 0.0;
+        const double ril = rinv[l];
+        __builtin_amdgcn_sched_barrier(0);
+        const double y = t[l] * ril;
+        t[l] = y;
+#pragma unroll
+        for (int i = 0; i < CB; ++i)
+          if (i > l) t[i] -= rowl[i] * y;
       }
-      __syncthreads();
+#pragma unroll
+      for (int l = 0; l < CB; ++l) Rs[l][c] = t[l];  // rows >= nb stay zero
     }
-    // trailing update on 4 x 4 tiles (ti <= tc) of the upper triangle
-    const int nT = (nt + 3) / 4;
-    const int ntiles = nT * (nT + 1) / 2;
-    for (int t = tid; t < ntiles; t += nthr) {
+    __syncthreads();
+    for (int e = tid; e < nb * nt; e += nthr) {
+      const int i = e / nt, c = e % nt;
+      A[(jb + i) * lda + c0 + c] = Rs[i][c];
+    }
+    // 3. trailing SYRK on upper-triangle tiles (ti <= tc); tile index t
+    //    enumerates row ti's tiles tc = ti .. T-1 in order
+    const int T = ntp >> 4;
+    const int ntiles = T * (T + 1) / 2;
+    for (int t = wave; t < ntiles; t += nwaves) {
       int ti = 0, rem = t;
-      while (rem >= nT - ti) {
-        rem -= nT - ti;
+      while (rem >= T - ti) {
+        rem -= T - ti;
         ++ti;
       }
       const int tc = ti + rem;
-      double acc[4][4];
+      double cv[4], av[CB / 4], bv[CB / 4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-      for (int l = 0; l < nb; ++l) {
-        double ri[4], rc[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const int i = ti * 4 + a, cc = tc * 4 + a;
-          ri[a] = (i < nt) ? Rs[l][i] : 0.0;
-          rc[a] = (cc < nt) ? Rs[l][cc] : 0.0;
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] += ri[a] * rc[b];
+      for (int r = 0; r < 4; ++r) {
+        const int i = ti * 16 + lk + 4 * r, cc = tc * 16 + lr;
+        cv[r] = (i < nt && cc < nt && cc >= i) ? A[(c0 + i) * lda + c0 + cc] : 0.0;
       }
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int q = 0; q < CB / 4; ++q) {
+        av[q] = Rs[4 * q + lk][ti * 16 + lr];
+        bv[q] = Rs[4 * q + lk][tc * 16 + lr];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int i = ti * 4 + a, cc = tc * 4 + b;
-          if (i < nt && cc < nt && cc >= i) A[(c0 + i) * lda + c0 + cc] -= acc[a][b];
-        }
+      for (int q = 0; q < CB / 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = ti * 16 + lk + 4 * r, cc = tc * 16 + lr;
+        if (i < nt && cc < nt && cc >= i) A[(c0 + i) * lda + c0 + cc] = cv[r] - acc[r];
+      }
     }
     __syncthreads();
   }
@@ -241,7 +234,7 @@ extern "C" int pamd_potrf_upper_f64(int64_t n, double* A, int64_t lda, int* info
   PAMD_ARG_CHECK(A != nullptr || n == 0, 2);
   PAMD_ARG_CHECK(lda >= n && lda >= 1, 3);
   PAMD_ARG_CHECK(info != nullptr, 4);
-  potrf_kernel<<<1, 512, 0, (hipStream_t)stream>>>(n, A, lda, info);
+  potrf_kernel<<<1, POTRF_THREADS, 0, (hipStream_t)stream>>>(n, A, lda, info);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

@@ -67,6 +67,15 @@ __device__ __forceinline__ float dpp_f32(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
 }
 
+// Returns x unchanged but opaque to the optimiser, at the point where it is
+// called: values derived from it are not hoisted out of the enclosing loop.
+// Used where loop-invariant lane masks would otherwise be precomputed for
+// every unrolled step and spilled.
+__device__ __forceinline__ int pamd_opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // value of v in lane `lane` (uniform index), for every lane
 __device__ __forceinline__ double wave_bcast(double v, int lane) {
   const long long b = __double_as_longlong(v);
