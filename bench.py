@@ -63,7 +63,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle timings")
     ap.add_argument("--ls-steps", type=int, default=10)
+    ap.add_argument("--legs", default="qb,ls,mixed", help="comma list of qb, ls, mixed")
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
               "multi-GPU path (see DESIGN.md for how the workload would shard).", file=sys.stderr)
@@ -141,6 +143,50 @@ def main():
     _, lg = spo(Al, bl, 0.0, 1e-12, 100, 7, logging=True)
     ls_iters = int(lg.errors.size - 1)
 
+    # ------------------------------------------------------------- mixed LS
+    mixed_extra = {}
+    if "mixed" in legs:
+        import oracle as orc
+        log("C3: generating A 262144 x 1024 (cond 1e8) on the host")
+        A3h, b3h, _ = mm.config_ls_mixed()
+        A3 = torch.from_numpy(A3h).to(dev)
+        b3 = torch.from_numpy(b3h).to(dev)
+        mixed = pa.SPOMixed(4, npieces=4, backend="device")
+
+        def mixed_step(seed):
+            out["MX"] = mixed(A3, b3, 0.0, 1e-12, 100, seed, logging=False)
+
+        log("C3: mixed-precision SPO (exact BF16 sketch, FP32 operator, FP64 refinement)")
+        t_mx = timed_steps(mixed_step, 3, 1, torch)
+        x_mx = out["MX"][0].cpu().numpy()
+        _, lgm = mixed(A3, b3, 0.0, 1e-12, 100, 7, logging=True)
+        plain3 = pa.SPO(pa.SkOpGA(backend="device"), 4)
+
+        def plain3_step(seed):
+            out["P3"] = plain3(A3, b3, 0.0, 1e-12, 100, seed, logging=False)
+
+        log("C3: plain FP64 SPO on the same problem")
+        t_p3 = timed_steps(plain3_step, 2, 1, torch)
+        x_p3 = out["P3"][0].cpu().numpy()
+        log("C3: backward errors in extended precision on the host")
+        eta_mx = orc.backward_error_extended(A3h, b3h, x_mx)
+        eta_p3 = orc.backward_error_extended(A3h, b3h, x_p3)
+        mixed_extra = {
+            "ls_mixed_workload": "A 262144x1024 f64, s geometric 1..1e-8, b = A x* + 1e-8 N(0,1); exact BF16 "
+                                 "Gaussian sketch d=4096 (4 BF16 pieces of A), FP64 QR, B = A R^-1 stored FP32, "
+                                 "FP64 refinement with CG on B^T B to eta <= 1e-12 or the FP64 floor",
+            "ls_mixed_solves_per_s": round(1.0 / t_mx, 3), "ls_mixed_ms": round(t_mx * 1e3, 3),
+            "ls_mixed_eta": eta_mx, "ls_mixed_refinement_etas": [float(e) for e in lgm.errors],
+            "ls_mixed_inner_cg_iters": int(lgm.inner_iterations),
+            "ls_mixed_phase_ms": {"sketch": round(lgm.time_sketch * 1e3, 3), "factor": round(lgm.time_factor * 1e3, 3),
+                                  "presolve": round(lgm.time_presolve * 1e3, 3),
+                                  "refine": round(lgm.time_iterate * 1e3, 3)},
+            "ls_plain_f64_c3_solves_per_s": round(1.0 / t_p3, 3), "ls_plain_f64_c3_ms": round(t_p3 * 1e3, 3),
+            "ls_plain_f64_c3_eta": eta_p3,
+            "ls_mixed_speedup_vs_plain_f64": round(t_p3 / t_mx, 3),
+        }
+        del A3, b3
+
     # ------------------------------------------------------------------ CPU
     cpu = None
     extra_cpu = {}
@@ -202,6 +248,7 @@ def main():
             "ls_workload": "SPO qr, A 20000x200 f64, s geometric 1..1e-6, Gaussian d=800 drawn on the "
                            "device, LSQR tol 1e-12, logging off",
             **extra_cpu,
+            **mixed_extra,
         },
     }
     print(json.dumps(line))

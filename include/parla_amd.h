@@ -239,6 +239,114 @@ int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, double* x_orig,
                       double* var, double* arnorms, pamd_lsqr_result* result,
                       void* workspace, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Mixed precision: exact BF16 sketch, compensated residual, CG operator      */
+/* ------------------------------------------------------------------------ */
+
+/* Quantised Gaussian sketch Q (rows x m, stored as BF16 bit patterns, row-major,
+ * ldq >= kp = m rounded up to a multiple of 256, ldq % 8 == 0): Q[i][k] =
+ * rint(32 z_ik) clipped to [-256, 256] with z the same Philox stream as
+ * pamd_gaussian_fill_f64 (counter offset + i*m + k); columns m..kp-1 are zero.
+ * The sketching operator is S = Q * 2^-5 / sqrt(min(rows, m)), the
+ * normalisation of utils/sketching.py:23 on an integer grid, so that the
+ * BF16 sketch product below is exact.  Replaces rng.normal of
+ * utils/sketching.py:24 for the mixed-precision sketch. */
+int pamd_gauss_q_bf16(int64_t rows, int64_t m, uint64_t seed, uint64_t offset,
+                      uint16_t* Q, int64_t ldq, void* stream);
+
+/* Error-free split of [A | b] (m x n, plus b as column n when b != NULL) into
+ * npieces (1..5) BF16 integer pieces per 256-row chunk and column:
+ * A[k][j] ~= 2^(e_cj - 8) sum_p 2^(-9p) a_p[k][j], |a_p| <= 256, with the
+ * exponent e_cj (max |A[chunk c, j]| < 2^e_cj) in exps[c * lde + j].
+ * pieces: npieces blocks of pstride elements, block p holding column j's
+ * chunk-padded K run at pieces + p*pstride + j*ldk (ldk >= kp, ldk % 8 == 0,
+ * pstride >= ncp*ldk, ncp = number of columns rounded up to 64).  First half
+ * of the exact sketch of A (drivers/least_squares.py:303,314 in mixed
+ * precision). */
+int pamd_ozaki_split_bf16(int64_t m, int64_t n, const double* A, int64_t lda,
+                          const double* b, int npieces, uint16_t* pieces,
+                          int64_t ldk, int64_t pstride, int* exps, int64_t lde,
+                          void* stream);
+
+/* out (d x ncols, FP64) = scale * Q @ [A | b] from the split above, on
+ * v_mfma_f32_16x16x32_bf16: every 256-long partial sum of integer products is
+ * exact in FP32 and is added in FP64 with its chunk scale.  The result equals
+ * scale * Q @ (A truncated to 9*npieces bits below each chunk-column maximum)
+ * up to the FP64 additions of the chunk sums.  Replaces S @ A and S @ b of
+ * drivers/least_squares.py:303,314. */
+int pamd_ozaki_sketch_f64(int64_t d, int64_t m, int64_t ncols, int npieces,
+                          const uint16_t* Q, int64_t ldq, const uint16_t* pieces,
+                          int64_t ldk, int64_t pstride, const int* exps,
+                          int64_t lde, double scale, double* out, int64_t ldo,
+                          void* stream);
+
+/* q = B^T (B p) in one read of B (m x n, n <= 1024, row-major; dtype 0: FP64,
+ * 1: FP32 storage), FP64 vectors and accumulation, deterministic order.  The
+ * operator of the refinement's inner CG on the preconditioned normal
+ * equations; no counterpart in the reference, whose solver is LSQR
+ * (comps/determiter/lsqr.py:98). */
+int pamd_normal_pass_workspace(int64_t m, int64_t n, int64_t* bytes);
+int pamd_normal_pass(int dtype, int64_t m, int64_t n, const void* B, int64_t ldb,
+                     const double* p, double* q, void* workspace, void* stream);
+
+/* Compensated residual, one read of A (m x n, n <= 1024): r = b - A x with
+ * error-free products and sums (Dot2), g = A^T r likewise, rss[0] = ||r||^2.
+ * r and rss may be NULL.  Replaces the FP64 residuals of the SPO log and
+ * presolve (drivers/least_squares.py:344-347, 361-366) where the backward
+ * error must be evaluated below the FP64 rounding floor of A x. */
+int pamd_residual_workspace(int64_t m, int64_t n, int64_t* bytes);
+int pamd_residual_f64(int64_t m, int64_t n, const double* A, int64_t lda,
+                      const double* x, const double* b, double* r, double* g,
+                      double* rss, void* workspace, void* stream);
+
+/* X (m x n) <- X R^{-1}, R upper n x n, for large m: recursive, 32-column
+ * substitution leaves and FP64 MFMA GEMM updates (backward stable like
+ * substitution, unlike multiplying by an explicit inverse).  Forms the
+ * explicitly preconditioned A R^{-1} that comps/preconditioning.py:16-41
+ * applies implicitly; workspace as for pamd_gemm_f64 with M = m. */
+int pamd_trsm_right_upper_rec_f64(int64_t m, int64_t n, const double* R,
+                                  int64_t ldr, double* X, int64_t ldx,
+                                  void* workspace, int64_t workspace_bytes,
+                                  void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Learned leverage-score sketch                                             */
+/* ------------------------------------------------------------------------ */
+
+/* The MLP surrogate 16 -> 64 -> 64 -> 1 (ReLU) maps a row's 16 pre-sketch
+ * features to the log of its normalised leverage score.  Weights wb are BF16
+ * bit patterns W1[16][64] | W2[64][64] | w3[64]; biases FP32 b1[64] | b2[64] |
+ * b3; products and sums FP32.  The flat FP32 parameter / gradient vector is
+ * W1 | b1 | W2 | b2 | w3 | b3 (pamd_mlp_nparams() entries).  No counterpart
+ * in the reference; it feeds the row-sampling operator that SkOpIN /
+ * sampling_operator (comps/sketchers/oblivious.py:75-82,
+ * utils/sketching.py:204-232) expose as an API slot. */
+int pamd_mlp_nparams(void);
+
+/* y[r] = MLP(X[r, 0:16]) for r < rows (X FP32 row-major, ldx >= 16). */
+int pamd_mlp_fwd(int64_t rows, const float* X, int64_t ldx, const uint16_t* wb,
+                 const float* bias, float* y, void* stream);
+
+/* grad (nparams + 1 FP32) = d/dparams of (1/rows_total) sum_r (MLP(X_r) - t_r)^2
+ * over these rows; grad[nparams] = the loss contribution.  Deterministic. */
+int pamd_mlp_bwd_workspace(int64_t rows, int64_t* bytes);
+int pamd_mlp_bwd(int64_t rows, int64_t rows_total, const float* X, int64_t ldx,
+                 const float* t, const uint16_t* wb, const float* bias,
+                 float* grad, void* workspace, void* stream);
+
+/* One Adam step (bias-corrected, step >= 1) on the FP32 masters w with
+ * moments m, v; then wb / bias are refreshed (BF16 round-to-nearest-even). */
+int pamd_mlp_adam(float* w, float* m, float* v, const float* grad, float lr,
+                  float beta1, float beta2, float eps, int64_t step,
+                  uint16_t* wb, float* bias, void* stream);
+
+/* out[r, :] = scale[r] * A[idx[r], :] (scale may be NULL: 1): the row-sampling
+ * sketch S @ A of utils/sketching.py:204-232 with the sampling weights of a
+ * leverage-score distribution. */
+int pamd_row_gather_f64(int64_t d, int64_t m, int64_t n, const int64_t* idx,
+                        const double* scale, const double* A, int64_t lda,
+                        double* out, int64_t ldo, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

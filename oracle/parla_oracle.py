@@ -22,6 +22,13 @@ Parity status of each function (how it is pinned, see tests/test_oracle_*.py):
 * rs1/rf1/qb1/svd1/evd1 -- exact low-rank recovery, orthonormality,
   B = Q^T A, power-method convergence (the reference's test families).
 * sampling_operator -- brute force (index gather == dense selection matrix).
+* quantize_q, ozaki_split, ozaki_product -- exact integer arithmetic: the
+  pieces reassemble A to within 2^-9P of each chunk-column maximum and the
+  product equals an int64 evaluation (tests/test_oracle.py).
+* residual_exact -- correctly rounded (exact products + math.fsum), checked
+  against exact rational arithmetic; residual_extended against it.
+* spo_mixed -- the FP64 least-squares solution (scipy lstsq) and the target
+  backward error on the reference's test families.
 """
 import time
 import warnings
@@ -37,7 +44,8 @@ __all__ = [
     "sampling_operator", "dim_checks", "sym_ortho", "lsqr", "DenseOp",
     "a_lift_precond", "pcss2", "SketchAndPrecondLog", "spo", "orth",
     "lu_stabilize", "rs1", "rf1", "qb1", "svd1", "evd1", "loglinear_fit",
-    "loglog_fit", "backward_error", "EPS",
+    "loglog_fit", "backward_error", "EPS", "quantize_q", "ozaki_split", "ozaki_product",
+    "residual_extended", "residual_exact", "backward_error_extended", "spo_mixed",
 ]
 
 
@@ -562,6 +570,160 @@ def evd1(A, k, tol, over, rng, qb):
     r = min(k, Q.shape[1], np.count_nonzero(alamb > 10 * EPS))
     order = np.argsort(-alamb)[:r]
     return Q @ U[:, order], lamb[order]
+
+
+# --------------------------------------------------------------------------
+# mixed precision: exact BF16 sketch and FP64 iterative refinement (no
+# counterpart in the reference; pinned by exact integer arithmetic, extended
+# precision residuals and the FP64 least-squares solution)
+# --------------------------------------------------------------------------
+
+OZ_CHUNK = 256
+
+
+def quantize_q(Z):
+    """Integer-grid Gaussian: clip(rint(32 Z), -256, 256)."""
+    return np.clip(np.rint(32.0 * np.asarray(Z, dtype=np.float64)), -256.0, 256.0)
+
+
+def ozaki_split(A, b=None, npieces=4):
+    """Split [A | b] per 256-row chunk and column into integer pieces:
+    A[k, j] = 2^(e_cj - 8) sum_p 2^(-9p) a_p[k, j] + remainder.
+    Returns (pieces (P, m, ncols) float64 integers, exps (nchunks, ncols) int)."""
+    M = np.asarray(A, dtype=np.float64)
+    if b is not None:
+        M = np.column_stack([M, np.asarray(b, dtype=np.float64)])
+    m, nc = M.shape
+    nch = -(-m // OZ_CHUNK)
+    exps = np.zeros((nch, nc), dtype=np.int64)
+    pieces = np.zeros((npieces, m, nc))
+    for c in range(nch):
+        blk = M[c * OZ_CHUNK:(c + 1) * OZ_CHUNK]
+        mx = np.max(np.abs(blk), axis=0)
+        e = np.where(mx > 0, np.frexp(mx)[1], 0)
+        exps[c] = e
+        y = np.ldexp(blk, (8 - e)[None, :])
+        for p in range(npieces):
+            a = np.rint(y)
+            pieces[p, c * OZ_CHUNK:(c + 1) * OZ_CHUNK] = a
+            y = (y - a) * 512.0
+    return pieces, exps
+
+
+def ozaki_product(Q, pieces, exps, scale):
+    """scale * Q @ (truncated [A | b]) evaluated as the device does: exact
+    integer chunk sums, added in FP64 in the order chunk-major, piece-minor."""
+    Qi = np.asarray(Q, dtype=np.int64)
+    P, m, nc = pieces.shape
+    d = Qi.shape[0]
+    acc = np.zeros((d, nc))
+    for c in range(exps.shape[0]):
+        sl = slice(c * OZ_CHUNK, min((c + 1) * OZ_CHUNK, m))
+        for p in range(P):
+            s = Qi[:, sl] @ pieces[p, sl].astype(np.int64)  # exact
+            acc += s.astype(np.float64) * np.ldexp(1.0, exps[c] - 8 - 9 * p)[None, :]
+    return scale * acc
+
+
+def _split26(v):
+    """Veltkamp split: v = hi + lo with hi, lo of at most 26 significant bits."""
+    c = v * 134217729.0  # 2^27 + 1
+    hi = c - (c - v)
+    return hi, v - hi
+
+
+def _exact_products(a, x):
+    """The four partial products whose sum is exactly a * x (elementwise)."""
+    ah, al = _split26(a)
+    xh, xl = _split26(x)
+    return ah * xh, ah * xl, al * xh, al * xl
+
+
+def residual_exact(A, x, b):
+    """(r, A^T r, ||r||^2) with r_i and (A^T r)_j correctly rounded: exact
+    products (Veltkamp/Dekker splitting) summed by math.fsum.  A^T r is the
+    correctly rounded product of A^T with the rounded r."""
+    import math
+    A = np.asarray(A, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    m, n = A.shape
+    parts = _exact_products(A, x[None, :])
+    r = np.empty(m)
+    for i in range(m):
+        r[i] = math.fsum([b[i]] + [-v for p in parts for v in p[i]])
+    parts = _exact_products(A, r[:, None])
+    g = np.array([math.fsum(v for p in parts for v in p[:, j]) for j in range(n)])
+    rss = math.fsum((r * r).tolist())
+    return r, g, rss
+
+
+def residual_extended(A, x, b):
+    """(r, A^T r, ||r||^2) in x87 extended precision (64-bit significand);
+    a quick checker, 2^11 times finer than FP64 but not exact."""
+    Al = np.asarray(A, dtype=np.longdouble)
+    r = np.asarray(b, dtype=np.longdouble) - Al @ np.asarray(x, dtype=np.longdouble)
+    g = Al.T @ r
+    return r.astype(np.float64), g.astype(np.float64), float(np.sum(r * r))
+
+
+def backward_error_extended(A, b, x):
+    """eta = ||A^T r|| / (||A||_F ||r||) with r and A^T r in extended precision."""
+    r, g, rss = residual_extended(A, x, b)
+    return float(np.linalg.norm(g) / (np.linalg.norm(A) * np.sqrt(rss)))
+
+
+def spo_mixed(A, b, tol, rng, sampling_factor=4, npieces=4, inner_tol=1e-6, max_inner=60,
+              max_refine=8):
+    """Restatement of the mixed-precision SPO: exact BF16-grid Gaussian sketch
+    of [A | b], Householder QR, B = A R^{-1} stored in FP32, then FP64
+    iterative refinement x <- x + R^{-1} CG(B^T B, R^{-T} A^T r(x)) with the
+    residuals in extended precision, until eta(x) <= tol or a step no longer
+    halves eta.  Returns the x with the smallest eta and the eta history."""
+    A = np.asarray(A, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    m, n = A.shape
+    d = dim_checks(sampling_factor, m, n)
+    rng = np.random.default_rng(rng)
+    Q = quantize_q(rng.standard_normal((d, m)))
+    pieces, exps = ozaki_split(A, b, npieces)
+    Mx = ozaki_product(Q, pieces, exps, 2.0 ** -5 / np.sqrt(min(d, m)))
+    Qf, R = la.qr(Mx[:, :n], mode='economic')
+    z = Qf.T @ Mx[:, n]
+    x = la.solve_triangular(R, z)
+    B32 = la.solve_triangular(R, A.T, trans='T').T.astype(np.float32)
+    B = B32.astype(np.float64)
+    anorm = np.linalg.norm(A)
+    etas = []
+    best = (np.inf, x)
+    for step in range(max_refine + 1):
+        r, g, rss = residual_extended(A, x, b)
+        eta = float(np.linalg.norm(g) / (anorm * np.sqrt(rss)))
+        etas.append(eta)
+        if eta < best[0]:
+            best = (eta, x)
+        # stop at the target, or once a step no longer halves eta (the FP64
+        # floor of eta for this problem: x itself is rounded to FP64)
+        if eta <= tol or (step > 0 and eta > 0.5 * etas[-2]) or step == max_refine:
+            break
+        rhs = la.solve_triangular(R, g, trans='T')
+        dz = np.zeros(n)
+        rv = rhs.copy()
+        pv = rv.copy()
+        rr = rr0 = rv @ rv
+        for _ in range(max_inner):
+            q = B.T @ (B @ pv)
+            alpha = rr / (pv @ q)
+            dz += alpha * pv
+            rv -= alpha * q
+            rr_new = rv @ rv
+            if rr_new <= inner_tol ** 2 * rr0:
+                break
+            pv = rv + (rr_new / rr) * pv
+            rr = rr_new
+        x = x + la.solve_triangular(R, dz)
+    x = best[1]
+    return x, np.array(etas)
 
 
 # --------------------------------------------------------------------------

@@ -64,6 +64,23 @@ SIGNATURES = {
     "pamd_potrf_upper_f64": (c_int, [c_i64, c_vp, c_i64, c_vp, c_vp]),
     "pamd_trtri_upper_f64": (c_int, [c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
     "pamd_trsm_right_upper_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
+    "pamd_gauss_q_bf16": (c_int, [c_i64, c_i64, ctypes.c_uint64, ctypes.c_uint64, c_vp, c_i64, c_vp]),
+    "pamd_ozaki_split_bf16": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_int, c_vp, c_i64, c_i64, c_vp, c_i64,
+                                      c_vp]),
+    "pamd_ozaki_sketch_f64": (c_int, [c_i64, c_i64, c_i64, c_int, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64,
+                                      c_dbl, c_vp, c_i64, c_vp]),
+    "pamd_normal_pass_workspace": (c_int, [c_i64, c_i64, ctypes.POINTER(c_i64)]),
+    "pamd_normal_pass": (c_int, [c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "pamd_residual_workspace": (c_int, [c_i64, c_i64, ctypes.POINTER(c_i64)]),
+    "pamd_residual_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "pamd_trsm_right_upper_rec_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
+    "pamd_mlp_nparams": (c_int, []),
+    "pamd_mlp_fwd": (c_int, [c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "pamd_mlp_bwd_workspace": (c_int, [c_i64, ctypes.POINTER(c_i64)]),
+    "pamd_mlp_bwd": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "pamd_mlp_adam": (c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float, c_i64, c_vp, c_vp, c_vp]),
+    "pamd_row_gather_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp]),
     "pamd_lsqr_workspace": (c_int, [c_i64, c_i64, c_i64, ctypes.POINTER(c_i64)]),
     "pamd_lsqr_run_f64": (c_int, [ctypes.POINTER(LsqrProblem), c_vp, c_vp, c_vp, c_vp,
                                   ctypes.POINTER(LsqrResult), c_vp, c_vp]),

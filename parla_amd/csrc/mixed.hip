@@ -1,0 +1,700 @@
+// Mixed-precision sketch-and-precondition kernels.
+//
+// 1. Exact BF16 sketch (error-free splitting, after Ozaki, Ogita, Oishi and
+//    Rump, "Error-free transformations of matrix multiplication by using fast
+//    routines of matrix multiplication", Numer. Algorithms 59, 2012).
+//    The Gaussian sketch is drawn directly on an integer grid,
+//        S = 2^-5 / sqrt(min(d, m)) * Q,   Q[i][k] = rint(32 z_ik)  (|Q| <= 256),
+//    so every entry of Q is an exact BF16 number.  A (and b, as one more
+//    column) is cut into chunks of 256 rows; in chunk c, column j is scaled by
+//    2^(8 - e_cj) so that it lies in (-256, 256) and split into P BF16 pieces
+//        A[k][j] = 2^(e_cj - 8) * sum_p 2^(-9p) a_p[k][j] + (remainder < 2^(e_cj - 9P)),
+//    every a_p an integer in [-256, 256].  A product Q[i][k] a_p[k][j] is an
+//    integer below 2^16 in magnitude, so a 256-term sum of them is below 2^24
+//    and v_mfma_f32_16x16x32_bf16 accumulates it in FP32 with no rounding at
+//    all.  Each chunk's FP32 sums are scaled by 2^(e_cj - 8 - 9p) and added to
+//    FP64 accumulators.  The only errors are the truncation of A after P
+//    pieces (relative 2^-9P to the chunk-column maximum) and the FP64
+//    additions of the chunk sums.
+// 2. q = B^T (B p) for an FP32- or FP64-stored B with FP64 vectors: one read
+//    of B per call (the operator of CG on the preconditioned normal equations).
+// 3. Compensated residual: r = b - A x and g = A^T r with error-free products
+//    and sums (Ogita, Rump and Oishi, "Accurate sum and dot product", SIAM J.
+//    Sci. Comput. 26, 2005), one read of A.
+// 4. Right triangular solve X R = A for large m, recursive: 32-column leaves
+//    by substitution (one thread per row), the off-diagonal blocks by the
+//    FP64 MFMA GEMM.
+#include "common.h"
+#include "rng.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int OZ_CHUNK = 256;       // K rows per exact FP32 accumulation
+constexpr int OZ_MAX_PIECES = 5;
+constexpr int OZ_TM = 128, OZ_TN = 64, OZ_TK = 64;
+constexpr int OZ_THREADS = 512;     // 8 waves, 4 (M) x 2 (N), 32 x 32 outputs each
+constexpr int OZ_LDS_PAD = 8;       // bf16 elements of padding per LDS row (16 B)
+
+__device__ __forceinline__ uint16_t bf16_bits_of_int(double a) {
+  // a is an integer with |a| <= 256: exact in BF16 (8 significant bits)
+  const float f = (float)a;
+  return (uint16_t)(__float_as_uint(f) >> 16);
+}
+
+// ---------------------------------------------------------------------------
+// quantised Gaussian, rows x kp (zeros for k >= m); entry (i, k < m) uses the
+// same Philox counter as the FP64 fill of a rows x m matrix, so the FP64 fill
+// times 32, rounded to the nearest integer, is this operator
+// ---------------------------------------------------------------------------
+__global__ void gauss_q_kernel(int64_t rows, int64_t m, int64_t kp, uint64_t seed, uint64_t offset,
+                               uint16_t* __restrict__ Q, int64_t ldq) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * kp) return;
+  const int64_t i = idx / kp, k = idx % kp;
+  uint16_t v = 0;
+  if (k < m) {
+    const uint64_t e = offset + (uint64_t)(i * m + k);
+    double z0, z1;
+    normal_pair(seed, e >> 1, z0, z1);
+    const double q = fmin(256.0, fmax(-256.0, rint(32.0 * ((e & 1) ? z1 : z0))));
+    v = bf16_bits_of_int(q);
+  }
+  Q[i * ldq + k] = v;
+}
+
+// ---------------------------------------------------------------------------
+// split: block (chunk c, 32 columns); thread (column jc, row group rg of 32)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double oz_load(int64_t k, int64_t j, int64_t m, int64_t n, const double* A, int64_t lda,
+                                          const double* b) {
+  if (k >= m) return 0.0;
+  if (j < n) return A[k * lda + j];
+  if (j == n && b != nullptr) return b[k];
+  return 0.0;
+}
+
+__global__ __launch_bounds__(256) void oz_split_kernel(int64_t m, int64_t n, const double* __restrict__ A, int64_t lda,
+                                                       const double* __restrict__ b, int P,
+                                                       uint16_t* __restrict__ pieces, int64_t ldk, int64_t pstride,
+                                                       int* __restrict__ exps, int64_t lde) {
+  __shared__ double smax[8][32];
+  __shared__ int sexp[32];
+  const int jc = threadIdx.x & 31, rg = threadIdx.x >> 5;
+  const int64_t c = blockIdx.x;
+  const int64_t j = (int64_t)blockIdx.y * 32 + jc;
+  const int64_t k0 = c * OZ_CHUNK + rg * 32;
+  double mx = 0.0;
+  for (int r = 0; r < 32; ++r) mx = fmax(mx, fabs(oz_load(k0 + r, j, m, n, A, lda, b)));
+  smax[rg][jc] = mx;
+  __syncthreads();
+  if (rg == 0) {
+    double t = 0.0;
+    for (int g = 0; g < 8; ++g) t = fmax(t, smax[g][jc]);
+    int e = 0;
+    if (t > 0.0) frexp(t, &e);  // t < 2^e
+    sexp[jc] = e;
+    if (j < lde) exps[c * lde + j] = e;
+  }
+  __syncthreads();
+  const int e = sexp[jc];
+  for (int p = 0; p < P; ++p) {
+    // pieces p of rows k0 .. k0+31, column j, in 4 stores of 8 BF16
+    uint16_t* dst = pieces + p * pstride + j * ldk + k0;
+    for (int q = 0; q < 4; ++q) {
+      uint16_t h[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        double y = ldexp(oz_load(k0 + 8 * q + i, j, m, n, A, lda, b), 8 - e);
+        double a = 0.0;
+        for (int pp = 0; pp <= p; ++pp) {  // piece p of y
+          a = rint(y);
+          y = (y - a) * 512.0;
+        }
+        h[i] = bf16_bits_of_int(a);
+      }
+      uint4 w;
+      w.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+      w.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+      w.z = (uint32_t)h[4] | ((uint32_t)h[5] << 16);
+      w.w = (uint32_t)h[6] | ((uint32_t)h[7] << 16);
+      *reinterpret_cast<uint4*>(dst + 8 * q) = w;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exact sketch GEMM: out[i][j] = scale * sum_c sum_p 2^(e_cj - 8 - 9p) Q[i, chunk c] . a_p[chunk c, j]
+// Q: d x kp (row-major, ldq), pieces: [P][ncp][ldk] (piece-major, column j's K run contiguous)
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(int64_t d, int64_t ncols, int64_t kp,
+                                                             const uint16_t* __restrict__ Q, int64_t ldq,
+                                                             const uint16_t* __restrict__ pieces, int64_t ldk,
+                                                             int64_t pstride, const int* __restrict__ exps,
+                                                             int64_t lde, double scale, double* __restrict__ out,
+                                                             int64_t ldo) {
+  constexpr int LDR = OZ_TK + OZ_LDS_PAD;
+  __shared__ __attribute__((aligned(16))) uint16_t Qs[2][OZ_TM][LDR];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][P][OZ_TN][LDR];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 3, wn = wave >> 2;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t i0 = (int64_t)blockIdx.x * OZ_TM, j0 = (int64_t)blockIdx.y * OZ_TN;
+
+  // global -> register staging: Q tile 128 x 64 = 1024 16-byte chunks (2 per
+  // thread), each piece tile 64 x 64 = 512 chunks (1 per thread)
+  uint4 rq[2], rb[P];
+  auto load_tile = [&](int64_t k0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int ch = tid + OZ_THREADS * h;
+      const int row = ch >> 3, kc = (ch & 7) * 8;
+      const int64_t gi = i0 + row;
+      rq[h] = (gi < d) ? *reinterpret_cast<const uint4*>(Q + gi * ldq + k0 + kc) : uint4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int col = tid >> 3, kc = (tid & 7) * 8;
+      rb[p] = *reinterpret_cast<const uint4*>(pieces + p * pstride + (j0 + col) * ldk + k0 + kc);
+    }
+  };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int ch = tid + OZ_THREADS * h;
+      *reinterpret_cast<uint4*>(&Qs[buf][ch >> 3][(ch & 7) * 8]) = rq[h];
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) *reinterpret_cast<uint4*>(&Bs[buf][p][tid >> 3][(tid & 7) * 8]) = rb[p];
+  };
+
+  f4 acc[P][2][2];
+  double acc64[2][2][4];
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int bq = 0; bq < 2; ++bq) acc[p][a][bq] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int bq = 0; bq < 2; ++bq)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[a][bq][r] = 0.0;
+
+  const int64_t ntiles = kp / OZ_TK;
+  constexpr int TPC = OZ_CHUNK / OZ_TK;  // K tiles per chunk
+  load_tile(0);
+  store_tile(0);
+  __syncthreads();
+  if (ntiles > 1) load_tile(OZ_TK);
+  int ecol[2] = {0, 0};
+  for (int64_t t = 0; t < ntiles; ++t) {
+    const int buf = (int)(t & 1);
+    if (t % TPC == 0) {  // exponents of this chunk, used at its flush
+      const int64_t cidx = t / TPC;
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) {
+        const int64_t j = j0 + 32 * wn + 16 * nt + lr;
+        ecol[nt] = (j < ncols) ? exps[cidx * lde + j] : 0;
+      }
+    }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x8 a[2], bv[P][2];
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+        a[mt] = *reinterpret_cast<const bf16x8*>(&Qs[buf][32 * wm + 16 * mt + lr][32 * ks + 8 * lk]);
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+          bv[p][nt] = *reinterpret_cast<const bf16x8*>(&Bs[buf][p][32 * wn + 16 * nt + lr][32 * ks + 8 * lk]);
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int nt = 0; nt < 2; ++nt)
+            acc[p][mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], bv[p][nt], acc[p][mt][nt], 0, 0, 0);
+    }
+    if (t + 1 < ntiles) {
+      store_tile(buf ^ 1);
+      if (t + 2 < ntiles) load_tile((t + 2) * OZ_TK);
+    }
+    if ((t + 1) % TPC == 0) {  // end of a chunk: exact FP32 sums -> FP64
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const double sc = ldexp(1.0, ecol[nt] - 8 - 9 * p);
+#pragma unroll
+          for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc64[mt][nt][r] += (double)acc[p][mt][nt][r] * sc;
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int nt = 0; nt < 2; ++nt) acc[p][mt][nt] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+    __syncthreads();
+  }
+  // D layout of v_mfma_f32_16x16x32_bf16: lane l, item r -> row 4 (l / 16) + r, col l % 16
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = i0 + 32 * wm + 16 * mt + 4 * lk + r;
+        const int64_t j = j0 + 32 * wn + 16 * nt + lr;
+        if (i < d && j < ncols) out[i * ldo + j] = scale * acc64[mt][nt][r];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// q = B^T (B p): row per wavefront, partials per workgroup, ordered reduce
+// ---------------------------------------------------------------------------
+constexpr int NP_THREADS = 256;
+constexpr int NP_MAX_GRID = 1024;
+
+template <typename T, int NPL>
+__global__ __launch_bounds__(NP_THREADS) void normal_pass_kernel(int64_t m, int64_t n, const T* __restrict__ B,
+                                                                 int64_t ldb, const double* __restrict__ p,
+                                                                 double* __restrict__ part) {
+  __shared__ double red[NP_THREADS / 64][64 * NPL];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t gw = (int64_t)gridDim.x * (NP_THREADS / 64);
+  double pr[NPL], acc[NPL];
+#pragma unroll
+  for (int q = 0; q < NPL; ++q) {
+    const int64_t c = lane + 64 * q;
+    pr[q] = (c < n) ? p[c] : 0.0;
+    acc[q] = 0.0;
+  }
+  for (int64_t row = (int64_t)blockIdx.x * (NP_THREADS / 64) + wave; row < m; row += gw) {
+    double a[NPL];
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      const int64_t c = lane + 64 * q;
+      a[q] = (c < n) ? (double)B[row * ldb + c] : 0.0;
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) s += a[q] * pr[q];
+    s = wave_sum(s);
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) acc[q] += s * a[q];
+  }
+#pragma unroll
+  for (int q = 0; q < NPL; ++q) red[wave][lane + 64 * q] = acc[q];
+  __syncthreads();
+  for (int64_t c = threadIdx.x; c < n; c += NP_THREADS) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < NP_THREADS / 64; ++w) t += red[w][c];
+    part[(int64_t)blockIdx.x * n + c] = t;
+  }
+}
+
+__global__ void reduce_partials_kernel(int64_t n, int64_t nparts, const double* __restrict__ part,
+                                       double* __restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double t = 0.0;
+  for (int64_t g = 0; g < nparts; ++g) t += part[g * n + c];
+  out[c] = t;
+}
+
+// ---------------------------------------------------------------------------
+// compensated residual
+// ---------------------------------------------------------------------------
+// Error-free transformations.  FP contraction is switched off in their
+// bodies: HIP contracts across statements by default and would fuse s + a * b
+// (the product from two_prod, the sum in two_sum) into one FMA, after which
+// the recovered error terms are no longer exact.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const double z = s - a;
+  e = (a - (s - z)) + (b - z);
+}
+
+// p + e == a * b exactly
+__device__ __forceinline__ void two_prod(double a, double b, double& p, double& e) {
+#pragma clang fp contract(off)
+  p = a * b;
+  e = fma(a, b, -p);
+}
+
+// a + b, never fused with a neighbouring product
+__device__ __forceinline__ double add_nc(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+
+// (s, c) pair sum over the wavefront, error-free at every step
+__device__ __forceinline__ void wave_sum2(double& s, double& c) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double so = __shfl_xor(s, off, 64);
+    const double co = __shfl_xor(c, off, 64);
+    double t, e;
+    two_sum(s, so, t, e);
+    s = t;
+    c = (c + co) + e;
+  }
+}
+
+template <int NPL>
+__global__ __launch_bounds__(NP_THREADS) void residual_kernel(int64_t m, int64_t n, const double* __restrict__ A,
+                                                              int64_t lda, const double* __restrict__ x,
+                                                              const double* __restrict__ b, double* __restrict__ r,
+                                                              double* __restrict__ part) {
+  // part layout per workgroup: [n sums][n compensations][2: sum r^2, sum of its errors]
+  __shared__ double red_s[NP_THREADS / 64][64 * NPL];
+  __shared__ double red_c[NP_THREADS / 64][64 * NPL];
+  __shared__ double red_r[NP_THREADS / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t gw = (int64_t)gridDim.x * (NP_THREADS / 64);
+  double xr[NPL], gs[NPL], gc[NPL];
+#pragma unroll
+  for (int q = 0; q < NPL; ++q) {
+    const int64_t c = lane + 64 * q;
+    xr[q] = (c < n) ? x[c] : 0.0;
+    gs[q] = 0.0;
+    gc[q] = 0.0;
+  }
+  double rr = 0.0;
+  for (int64_t row = (int64_t)blockIdx.x * (NP_THREADS / 64) + wave; row < m; row += gw) {
+    double a[NPL];
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      const int64_t c = lane + 64 * q;
+      a[q] = (c < n) ? A[row * lda + c] : 0.0;
+    }
+    double s = 0.0, cs = 0.0;
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      double pq, ep, t, e;
+      two_prod(a[q], xr[q], pq, ep);
+      two_sum(s, pq, t, e);
+      s = t;
+      cs = add_nc(cs, add_nc(ep, e));
+    }
+    wave_sum2(s, cs);
+    // r_i = b_i - (s + cs)
+    double t, e;
+    two_sum(b[row], -s, t, e);
+    const double ri = add_nc(t, add_nc(e, -cs));
+    if (lane == 0 && r != nullptr) r[row] = ri;
+    rr += ri * ri;
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      double pq, ep, u, f;
+      two_prod(ri, a[q], pq, ep);
+      two_sum(gs[q], pq, u, f);
+      gs[q] = u;
+      gc[q] = add_nc(gc[q], add_nc(ep, f));
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NPL; ++q) {
+    red_s[wave][lane + 64 * q] = gs[q];
+    red_c[wave][lane + 64 * q] = gc[q];
+  }
+  if (lane == 0) red_r[wave] = rr;
+  __syncthreads();
+  double* pw = part + (int64_t)blockIdx.x * (2 * n + 2);
+  for (int64_t c = threadIdx.x; c < n; c += NP_THREADS) {
+    double s = 0.0, cc = 0.0;
+#pragma unroll
+    for (int w = 0; w < NP_THREADS / 64; ++w) {
+      double t, e;
+      two_sum(s, red_s[w][c], t, e);
+      s = t;
+      cc = add_nc(cc, add_nc(red_c[w][c], e));
+    }
+    pw[c] = s;
+    pw[n + c] = cc;
+  }
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NP_THREADS / 64; ++w) t += red_r[w];
+    pw[2 * n] = t;
+    pw[2 * n + 1] = 0.0;
+  }
+}
+
+__global__ void residual_reduce_kernel(int64_t n, int64_t nparts, const double* __restrict__ part,
+                                       double* __restrict__ g, double* __restrict__ rss) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) {
+    double s = 0.0, cc = 0.0;
+    for (int64_t q = 0; q < nparts; ++q) {
+      double t, e;
+      two_sum(s, part[q * (2 * n + 2) + c], t, e);
+      s = t;
+      cc = add_nc(cc, add_nc(part[q * (2 * n + 2) + n + c], e));
+    }
+    g[c] = add_nc(s, cc);
+  } else if (c == n && rss != nullptr) {
+    double t = 0.0;
+    for (int64_t q = 0; q < nparts; ++q) t += part[q * (2 * n + 2) + 2 * n];
+    rss[0] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// TRSM leaf: X (m x nb, nb <= 64) <- X R^{-1}, R upper nb x nb; thread per row
+// ---------------------------------------------------------------------------
+constexpr int LEAF = 32;
+
+// FULL: nb == LEAF, rows moved straight between memory and registers; else the
+// partial last leaf goes through an LDS copy (a register array may only be
+// indexed with compile-time indices, and nb is a run-time value)
+template <bool FULL>
+__global__ __launch_bounds__(256) void trsm_leaf_kernel(int64_t m, int nb, const double* __restrict__ R, int64_t ldr,
+                                                        double* __restrict__ X, int64_t ldx) {
+  __shared__ double Rl[LEAF][LEAF + 1];
+  __shared__ double rinv[LEAF];
+  __shared__ double xs[FULL ? 1 : 256][LEAF + 1];
+  for (int e = threadIdx.x; e < LEAF * LEAF; e += blockDim.x) {
+    const int i = e / LEAF, j = e % LEAF;
+    const double v = (i < nb && j < nb) ? R[(int64_t)i * ldr + j] : ((i == j) ? 1.0 : 0.0);
+    Rl[i][j] = v;
+    if (i == j) rinv[i] = 1.0 / v;
+  }
+  __syncthreads();
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= m) return;
+  double* xr = X + row * ldx;
+  double x[LEAF];
+  if constexpr (FULL) {
+#pragma unroll
+    for (int j = 0; j < LEAF; ++j) x[j] = xr[j];
+  } else {
+    for (int j = 0; j < LEAF; ++j) xs[threadIdx.x][j] = (j < nb) ? xr[j] : 0.0;
+#pragma unroll
+    for (int j = 0; j < LEAF; ++j) x[j] = xs[threadIdx.x][j];
+  }
+#pragma unroll
+  for (int j = 0; j < LEAF; ++j) {
+    double rowj[LEAF];
+#pragma unroll
+    for (int i = 0; i < LEAF; ++i) rowj[i] = (i > j) ? Rl[j][i] : 0.0;
+    const double rj = rinv[j];
+    __builtin_amdgcn_sched_barrier(0);  // the row's LDS reads in one batch
+    const double y = x[j] * rj;
+    x[j] = y;
+#pragma unroll
+    for (int i = 0; i < LEAF; ++i)
+      if (i > j) x[i] -= rowj[i] * y;
+  }
+  if constexpr (FULL) {
+#pragma unroll
+    for (int j = 0; j < LEAF; ++j) xr[j] = x[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < LEAF; ++j) xs[threadIdx.x][j] = x[j];
+    for (int j = 0; j < nb; ++j) xr[j] = xs[threadIdx.x][j];
+  }
+}
+
+int trsm_rec(int64_t m, int64_t n, const double* R, int64_t ldr, double* X, int64_t ldx, void* ws, int64_t wsb,
+             hipStream_t st) {
+  if (n <= LEAF) {
+    if (n == LEAF)
+      trsm_leaf_kernel<true><<<(unsigned)pamd_cdiv(m, 256), 256, 0, st>>>(m, (int)n, R, ldr, X, ldx);
+    else
+      trsm_leaf_kernel<false><<<(unsigned)pamd_cdiv(m, 256), 256, 0, st>>>(m, (int)n, R, ldr, X, ldx);
+    PAMD_LAUNCH_CHECK();
+    return PAMD_OK;
+  }
+  const int64_t n1 = ((n / 2 + LEAF - 1) / LEAF) * LEAF;
+  int rc = trsm_rec(m, n1, R, ldr, X, ldx, ws, wsb, st);
+  if (rc) return rc;
+  // X2 -= X1 R12
+  rc = pamd_gemm_f64(0, 0, m, n - n1, n1, -1.0, X, ldx, R + n1, ldr, 1.0, X + n1, ldx, ws, wsb, st);
+  if (rc) return rc;
+  return trsm_rec(m, n - n1, R + n1 * ldr + n1, ldr, X + n1, ldx, ws, wsb, st);
+}
+
+inline int64_t np_grid(int64_t m) { return imax64(1, imin64(NP_MAX_GRID, pamd_cdiv(m, 64))); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int pamd_gauss_q_bf16(int64_t rows, int64_t m, uint64_t seed, uint64_t offset, uint16_t* Q, int64_t ldq,
+                                 void* stream) {
+  PAMD_ARG_CHECK(rows >= 0, 1);
+  PAMD_ARG_CHECK(m >= 0, 2);
+  const int64_t kp = pamd_cdiv(m, OZ_CHUNK) * OZ_CHUNK;
+  PAMD_ARG_CHECK(Q != nullptr || rows * kp == 0, 5);
+  PAMD_ARG_CHECK(ldq >= kp && ldq % 8 == 0, 6);
+  if (rows * kp == 0) return PAMD_OK;
+  gauss_q_kernel<<<(unsigned)pamd_cdiv(rows * kp, 256), 256, 0, (hipStream_t)stream>>>(rows, m, kp, seed, offset, Q,
+                                                                                       ldq);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_ozaki_split_bf16(int64_t m, int64_t n, const double* A, int64_t lda, const double* b,
+                                     int npieces, uint16_t* pieces, int64_t ldk, int64_t pstride, int* exps,
+                                     int64_t lde, void* stream) {
+  const int64_t ncols = n + (b != nullptr ? 1 : 0);
+  const int64_t kp = pamd_cdiv(m, OZ_CHUNK) * OZ_CHUNK;
+  const int64_t ncp = pamd_cdiv(ncols, OZ_TN) * OZ_TN;
+  PAMD_ARG_CHECK(m >= 0, 1);
+  PAMD_ARG_CHECK(n >= 0, 2);
+  PAMD_ARG_CHECK(A != nullptr || m * n == 0, 3);
+  PAMD_ARG_CHECK(lda >= n && lda >= 1, 4);
+  PAMD_ARG_CHECK(npieces >= 1 && npieces <= OZ_MAX_PIECES, 6);
+  PAMD_ARG_CHECK(pieces != nullptr || kp * ncp == 0, 7);
+  PAMD_ARG_CHECK(ldk >= kp && ldk % 8 == 0, 8);
+  PAMD_ARG_CHECK(pstride >= ncp * ldk, 9);
+  PAMD_ARG_CHECK(exps != nullptr || kp == 0, 10);
+  PAMD_ARG_CHECK(lde >= ncols, 11);
+  if (kp == 0 || ncp == 0) return PAMD_OK;
+  dim3 grid((unsigned)(kp / OZ_CHUNK), (unsigned)(ncp / 32));
+  oz_split_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(m, n, A, lda, b, npieces, pieces, ldk, pstride, exps, lde);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_ozaki_sketch_f64(int64_t d, int64_t m, int64_t ncols, int npieces, const uint16_t* Q,
+                                     int64_t ldq, const uint16_t* pieces, int64_t ldk, int64_t pstride,
+                                     const int* exps, int64_t lde, double scale, double* out, int64_t ldo,
+                                     void* stream) {
+  const int64_t kp = pamd_cdiv(m, OZ_CHUNK) * OZ_CHUNK;
+  const int64_t ncp = pamd_cdiv(ncols, OZ_TN) * OZ_TN;
+  PAMD_ARG_CHECK(d >= 0, 1);
+  PAMD_ARG_CHECK(m >= 0, 2);
+  PAMD_ARG_CHECK(ncols >= 0, 3);
+  PAMD_ARG_CHECK(npieces >= 1 && npieces <= OZ_MAX_PIECES, 4);
+  PAMD_ARG_CHECK(Q != nullptr || d * kp == 0, 5);
+  PAMD_ARG_CHECK(ldq >= kp && ldq % 8 == 0, 6);
+  PAMD_ARG_CHECK(pieces != nullptr || kp * ncp == 0, 7);
+  PAMD_ARG_CHECK(ldk >= kp && ldk % 8 == 0, 8);
+  PAMD_ARG_CHECK(pstride >= ncp * ldk, 9);
+  PAMD_ARG_CHECK(exps != nullptr || kp == 0, 10);
+  PAMD_ARG_CHECK(lde >= ncols, 11);
+  PAMD_ARG_CHECK(out != nullptr || d * ncols == 0, 13);
+  PAMD_ARG_CHECK(ldo >= ncols && ldo >= 1, 14);
+  if (d == 0 || ncols == 0) return PAMD_OK;
+  if (kp == 0) {
+    return PAMD_ERR_UNSUPPORTED;  // m == 0: nothing to sketch; the caller zeroes `out`
+  }
+  dim3 grid((unsigned)pamd_cdiv(d, OZ_TM), (unsigned)(ncp / OZ_TN));
+  hipStream_t st = (hipStream_t)stream;
+  switch (npieces) {
+    case 1: oz_gemm_kernel<1><<<grid, OZ_THREADS, 0, st>>>(d, ncols, kp, Q, ldq, pieces, ldk, pstride, exps, lde, scale, out, ldo); break;
+    case 2: oz_gemm_kernel<2><<<grid, OZ_THREADS, 0, st>>>(d, ncols, kp, Q, ldq, pieces, ldk, pstride, exps, lde, scale, out, ldo); break;
+    case 3: oz_gemm_kernel<3><<<grid, OZ_THREADS, 0, st>>>(d, ncols, kp, Q, ldq, pieces, ldk, pstride, exps, lde, scale, out, ldo); break;
+    case 4: oz_gemm_kernel<4><<<grid, OZ_THREADS, 0, st>>>(d, ncols, kp, Q, ldq, pieces, ldk, pstride, exps, lde, scale, out, ldo); break;
+    default: oz_gemm_kernel<5><<<grid, OZ_THREADS, 0, st>>>(d, ncols, kp, Q, ldq, pieces, ldk, pstride, exps, lde, scale, out, ldo); break;
+  }
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_normal_pass_workspace(int64_t m, int64_t n, int64_t* bytes) {
+  PAMD_ARG_CHECK(m >= 0, 1);
+  PAMD_ARG_CHECK(n >= 0 && n <= 1024, 2);
+  PAMD_ARG_CHECK(bytes != nullptr, 3);
+  *bytes = np_grid(m) * n * (int64_t)sizeof(double);
+  return PAMD_OK;
+}
+
+extern "C" int pamd_normal_pass(int dtype, int64_t m, int64_t n, const void* B, int64_t ldb, const double* p,
+                                double* q, void* workspace, void* stream) {
+  PAMD_ARG_CHECK(dtype == 0 || dtype == 1, 1);
+  PAMD_ARG_CHECK(m >= 0, 2);
+  PAMD_ARG_CHECK(n >= 1 && n <= 1024, 3);
+  PAMD_ARG_CHECK(B != nullptr || m == 0, 4);
+  PAMD_ARG_CHECK(ldb >= n, 5);
+  PAMD_ARG_CHECK(p != nullptr, 6);
+  PAMD_ARG_CHECK(q != nullptr, 7);
+  PAMD_ARG_CHECK(workspace != nullptr, 8);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t grid = np_grid(m);
+  double* part = (double*)workspace;
+  const int npl = (int)pamd_cdiv(n, 64);
+#define PAMD_NP(T, NPL) normal_pass_kernel<T, NPL><<<(unsigned)grid, NP_THREADS, 0, st>>>(m, n, (const T*)B, ldb, p, part)
+#define PAMD_NP_T(T)                 \
+  if (npl <= 1) PAMD_NP(T, 1);       \
+  else if (npl <= 2) PAMD_NP(T, 2);  \
+  else if (npl <= 4) PAMD_NP(T, 4);  \
+  else if (npl <= 8) PAMD_NP(T, 8);  \
+  else PAMD_NP(T, 16);
+  if (dtype == 0) {
+    PAMD_NP_T(double)
+  } else {
+    PAMD_NP_T(float)
+  }
+#undef PAMD_NP_T
+#undef PAMD_NP
+  PAMD_LAUNCH_CHECK();
+  reduce_partials_kernel<<<(unsigned)pamd_cdiv(n, 256), 256, 0, st>>>(n, grid, part, q);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_residual_workspace(int64_t m, int64_t n, int64_t* bytes) {
+  PAMD_ARG_CHECK(m >= 0, 1);
+  PAMD_ARG_CHECK(n >= 0 && n <= 1024, 2);
+  PAMD_ARG_CHECK(bytes != nullptr, 3);
+  *bytes = np_grid(m) * (2 * n + 2) * (int64_t)sizeof(double);
+  return PAMD_OK;
+}
+
+extern "C" int pamd_residual_f64(int64_t m, int64_t n, const double* A, int64_t lda, const double* x,
+                                 const double* b, double* r, double* g, double* rss, void* workspace,
+                                 void* stream) {
+  PAMD_ARG_CHECK(m >= 1, 1);
+  PAMD_ARG_CHECK(n >= 1 && n <= 1024, 2);
+  PAMD_ARG_CHECK(A != nullptr, 3);
+  PAMD_ARG_CHECK(lda >= n, 4);
+  PAMD_ARG_CHECK(x != nullptr, 5);
+  PAMD_ARG_CHECK(b != nullptr, 6);
+  PAMD_ARG_CHECK(g != nullptr, 8);
+  PAMD_ARG_CHECK(workspace != nullptr, 10);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t grid = np_grid(m);
+  double* part = (double*)workspace;
+  const int npl = (int)pamd_cdiv(n, 64);
+#define PAMD_RK(NPL) residual_kernel<NPL><<<(unsigned)grid, NP_THREADS, 0, st>>>(m, n, A, lda, x, b, r, part)
+  if (npl <= 1) PAMD_RK(1);
+  else if (npl <= 2) PAMD_RK(2);
+  else if (npl <= 4) PAMD_RK(4);
+  else if (npl <= 8) PAMD_RK(8);
+  else PAMD_RK(16);
+#undef PAMD_RK
+  PAMD_LAUNCH_CHECK();
+  residual_reduce_kernel<<<(unsigned)pamd_cdiv(n + 1, 256), 256, 0, st>>>(n, grid, part, g, rss);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_trsm_right_upper_rec_f64(int64_t m, int64_t n, const double* R, int64_t ldr, double* X,
+                                             int64_t ldx, void* workspace, int64_t workspace_bytes, void* stream) {
+  PAMD_ARG_CHECK(m >= 0, 1);
+  PAMD_ARG_CHECK(n >= 0, 2);
+  PAMD_ARG_CHECK(R != nullptr || n == 0, 3);
+  PAMD_ARG_CHECK(ldr >= n && ldr >= 1, 4);
+  PAMD_ARG_CHECK(X != nullptr || m * n == 0, 5);
+  PAMD_ARG_CHECK(ldx >= n && ldx >= 1, 6);
+  if (m == 0 || n == 0) return PAMD_OK;
+  return trsm_rec(m, n, R, ldr, X, ldx, workspace, workspace_bytes, (hipStream_t)stream);
+}

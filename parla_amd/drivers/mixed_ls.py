@@ -1,0 +1,155 @@
+"""Mixed-precision sketch-and-precondition least squares with FP64 refinement.
+
+Same call contract as SPO (drivers/least_squares.py:206-369 of the reference):
+``x, log = SPOMixed(...)(A, b, delta, tol, iter_lim, rng)``, with ``tol`` read
+as the target backward error eta(x) = ||A^T r|| / (||A||_F ||r||).
+
+  1. Sketch.  S = 2^-5 / sqrt(d) * Q with Q an integer-valued Gaussian
+     (|Q| <= 256, exact in BF16).  [A | b] is split into `npieces` BF16 integer
+     pieces per 256-row chunk, and [S A | S b] is formed on BF16 MFMA with
+     exact FP32 partial sums and FP64 accumulation: it equals S times A
+     truncated to 9 * npieces bits below each chunk-column maximum.
+  2. Factor.  Householder QR of S A in FP64; z = Q^T S b.
+  3. Precondition explicitly.  B = A R^{-1} in FP64 by a recursive MFMA TRSM,
+     stored in FP32 (kappa(B) is about 3, so FP32 storage costs nothing in
+     the refinement's contraction rate).
+  4. Refine in FP64.  x0 = R^{-1} z; then, until eta(x) <= tol or a step no
+     longer halves eta (the FP64 floor: x itself is rounded to FP64),
+         g  = A^T (b - A x)                 compensated FP64, one read of A
+         dz = CG(B^T B, R^{-T} g)            FP32 B, FP64 vectors, one read of B
+         x <- x + R^{-1} dz                   FP64 triangular solves.
+     The fixed point is A^T r = 0 evaluated with error-free transformations,
+     so the FP32 operator only sets the contraction rate, not the accuracy.
+
+The log's ``errors`` are the backward errors after each refinement step.
+"""
+import time
+import warnings
+
+import numpy as np
+import torch
+
+from parla_amd import ops
+from parla_amd.comps.determiter.logging import SketchAndPrecondLog
+from parla_amd.drivers.least_squares import OverLstsqSolver, dim_checks
+
+
+def _dot(a, b):
+    return torch.dot(a, b)
+
+
+def cg_normal(op, rhs, max_iter, tol, check_every=8):
+    """z ~= (B^T B)^{-1} rhs by conjugate gradients; scalars stay on the device,
+    one host check of ||r|| every `check_every` iterations."""
+    z = torch.zeros_like(rhs)
+    r = rhs.clone()
+    p = r.clone()
+    rr = _dot(r, r)
+    stop = (tol * tol) * rr
+    it = 0
+    while it < max_iter:
+        q = op(p)
+        alpha = rr / _dot(p, q)
+        z.add_(alpha * p)
+        r.sub_(alpha * q)
+        rr_new = _dot(r, r)
+        p = r + (rr_new / rr) * p
+        rr = rr_new
+        it += 1
+        if it % check_every == 0 and bool((rr <= stop).item()):
+            break
+    return z, it
+
+
+class SPOMixed(OverLstsqSolver):
+    """Sketch-and-precondition with an exact BF16 sketch, an FP32-stored
+    preconditioned operator and FP64 iterative refinement to ``tol``."""
+
+    def __init__(self, sampling_factor=4, npieces=4, inner_tol=1e-6, max_inner=64, max_refine=10,
+                 backend="numpy"):
+        if not 1 <= npieces <= 5:
+            raise ValueError("npieces must be in 1..5")
+        self.sampling_factor = sampling_factor
+        self.npieces = npieces
+        self.inner_tol = inner_tol
+        self.max_inner = max_inner
+        self.max_refine = max_refine
+        self.backend = backend
+
+    def __call__(self, A, b, delta, tol, iter_lim, rng, logging=True):
+        if delta != 0:
+            raise NotImplementedError("SPOMixed solves the unregularised problem (delta = 0)")
+        At = ops.as_device(A).contiguous()
+        bt = ops.as_device(b).reshape(-1).contiguous()
+        m, n = At.shape
+        if n > 1024:
+            raise NotImplementedError("SPOMixed supports n <= 1024")
+        d = dim_checks(self.sampling_factor, m, n)
+        rng = np.random.default_rng(rng)
+        dev = At.device
+        log = SketchAndPrecondLog()
+
+        def clock():
+            if logging:
+                torch.cuda.synchronize(dev)
+                return time.time()
+            return 0.0
+
+        tic = clock()
+        if self.backend == "numpy":
+            Q = ops.quantize_q(rng.standard_normal((d, m)))
+        else:
+            Q = ops.gauss_q(d, m, int(rng.integers(0, 2 ** 63)), dev)
+        split = ops.OzakiSplit(At, bt, self.npieces)
+        Mx = ops.ozaki_sketch(Q, split, 2.0 ** -5 / np.sqrt(min(d, m)))
+        del split, Q
+        log.time_sketch = clock() - tic
+
+        tic = clock()
+        fac = ops.geqrf_(Mx, n)
+        R = torch.triu(fac.R_view).contiguous()
+        z = Mx[:n, n].clone()
+        B = At.clone()
+        ops.trsm_rec_(B, R)
+        B32 = B.to(torch.float32)
+        del B
+        log.time_factor = clock() - tic
+
+        tic = clock()
+        x = ops.trsv_(R, z)
+        res = ops.Residual(At)
+        normal = ops.NormalOp(B32)
+        anorm = float(ops.nrm2(At.reshape(-1)).item())
+        log.time_presolve = clock() - tic
+
+        tic = clock()
+        etas = []
+        inner_total = 0
+        best_eta, best_x = np.inf, None
+        for step in range(self.max_refine + 1):
+            g, rss, _ = res(x, bt)
+            gn = ops.nrm2(g)
+            vals = torch.cat([gn.reshape(1), rss.reshape(1)]).cpu().numpy()  # one host read per step
+            rn = np.sqrt(vals[1])
+            eta = float(vals[0] / (anorm * rn)) if rn > 0 else 0.0
+            etas.append(eta)
+            if eta < best_eta:
+                best_eta, best_x = eta, x.clone()
+            # stop at the target, or once a step no longer halves eta: x is
+            # then at the FP64 floor of this problem's backward error
+            if eta <= tol or (step > 0 and eta > 0.5 * etas[-2]) or step == self.max_refine:
+                break
+            rhs = ops.trsv_(R, g, trans=True)
+            dz, its = cg_normal(normal, rhs, self.max_inner, self.inner_tol)
+            inner_total += its
+            x.add_(ops.trsv_(R, dz))
+        x = best_x
+        if best_eta > tol:
+            warnings.warn(f"SPOMixed: backward error {best_eta:.3e} stays above tol {tol:.1e} "
+                          f"(the FP64 floor of this problem, or max_refine = {self.max_refine} reached)")
+        log.time_iterate = clock() - tic
+        log.errors = np.array(etas)
+        log.inner_iterations = inner_total
+        log.refinement_steps = len(etas) - 1
+        log.error_desc = "backward error ||A^T r|| / (||A||_F ||r||) after each FP64 refinement step"
+        return ops.like_input(x, A), log

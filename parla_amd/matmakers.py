@@ -189,6 +189,11 @@ def config_ls_plain(m=20000, n=200):
     return ls_problem(m, n, 1e6, 1e-8, np.random.default_rng(data_seed(1)))
 
 
+def config_ls_mixed(m=262144, n=1024):
+    """Mixed-precision LS: A 262144x1024, s log-spaced from 1 to 1e-8 (cond 1e8)."""
+    return ls_problem(m, n, 1e8, 1e-8, np.random.default_rng(data_seed(3)))
+
+
 def config_qb(m=8192, n=4096, rank_hi=512):
     """QB -> SVD input: A = U diag(s) V^T + 1e-8 N(0,1), s_i = i^-2 (i <= 512), else 1e-6."""
     rng = np.random.default_rng(data_seed(2))

@@ -328,6 +328,127 @@ def gaussian_fill(rows, cols, scale, seed, dev, offset=0):
     return out
 
 
+# ---------------------------------------------------------------------------
+# mixed precision
+# ---------------------------------------------------------------------------
+
+OZ_CHUNK = 256
+OZ_TN = 64
+
+
+def _round_up(x, q):
+    return ((x + q - 1) // q) * q
+
+
+def gauss_q(rows, m, seed, dev, offset=0):
+    """Quantised Gaussian Q (rows x kp int-valued BF16, kp = m rounded up to
+    256, zeros beyond m): Q = rint(32 z) for the Philox stream of gaussian_fill."""
+    kp = _round_up(max(m, 1), OZ_CHUNK)
+    Q = torch.empty((rows, kp), dtype=torch.bfloat16, device=dev)
+    check(lib().pamd_gauss_q_bf16(rows, m, seed & (2 ** 64 - 1), offset, _p(Q), kp, stream(dev)),
+          "pamd_gauss_q_bf16")
+    return Q
+
+
+def quantize_q(Z, m=None):
+    """Q = clip(rint(32 Z), -256, 256) as a chunk-padded BF16 tensor (the host
+    draw of gauss_q: numpy normals -> the same integer grid)."""
+    Z = as_device(Z)
+    rows, mm = Z.shape
+    kp = _round_up(max(mm, 1), OZ_CHUNK)
+    Q = torch.zeros((rows, kp), dtype=torch.bfloat16, device=Z.device)
+    Q[:, :mm] = torch.clamp(torch.round(32.0 * Z), -256.0, 256.0).to(torch.bfloat16)
+    return Q
+
+
+class OzakiSplit:
+    """[A | b] cut into `npieces` BF16 integer pieces per 256-row chunk."""
+
+    def __init__(self, A, b=None, npieces=4):
+        m, n = A.shape
+        if A.stride(1) != 1:
+            A = A.contiguous()
+        self.m, self.n, self.npieces = m, n, npieces
+        self.ncols = n + (1 if b is not None else 0)
+        self.kp = _round_up(max(m, 1), OZ_CHUNK)
+        self.ncp = _round_up(self.ncols, OZ_TN)
+        self.pstride = self.ncp * self.kp
+        dev = A.device
+        self.pieces = torch.empty(npieces * self.pstride, dtype=torch.bfloat16, device=dev)
+        self.exps = torch.empty((self.kp // OZ_CHUNK, self.ncols), dtype=torch.int32, device=dev)
+        bb = None if b is None else b.contiguous()
+        check(lib().pamd_ozaki_split_bf16(m, n, _p(A), max(A.stride(0), n, 1), _p(bb), npieces, _p(self.pieces),
+                                          self.kp, self.pstride, _p(self.exps), self.ncols, stream(dev)),
+              "pamd_ozaki_split_bf16")
+
+    def piece(self, p):
+        """Piece p as an (ncols x kp) tensor (column j's K run in row j)."""
+        return self.pieces[p * self.pstride:(p + 1) * self.pstride].view(self.ncp, self.kp)[:self.ncols]
+
+
+def ozaki_sketch(Q, split, scale, out=None):
+    """out (d x ncols) = scale * Q @ [A | b] from the exact BF16 products."""
+    d = Q.shape[0]
+    if out is None:
+        out = torch.empty((d, split.ncols), dtype=F64, device=Q.device)
+    with _Timed("ozaki_sketch", 2.0 * d * split.kp * split.ncp * split.npieces,
+                2.0 * d * split.kp + 2.0 * split.npieces * split.kp * split.ncp + 8.0 * d * split.ncols):
+        check(lib().pamd_ozaki_sketch_f64(d, split.m, split.ncols, split.npieces, _p(Q), Q.stride(0),
+                                          _p(split.pieces), split.kp, split.pstride, _p(split.exps), split.ncols,
+                                          float(scale), _p(out), out.stride(0), stream(Q.device)),
+              "pamd_ozaki_sketch_f64")
+    return out
+
+
+class NormalOp:
+    """q = B^T (B p) for an FP32- or FP64-stored B, FP64 vectors."""
+
+    def __init__(self, B):
+        if B.dtype not in (torch.float32, F64) or B.stride(1) != 1:
+            raise ValueError("NormalOp needs a row-major float32 or float64 matrix")
+        self.B = B
+        m, n = B.shape
+        self.ws = workspace(_query(lib().pamd_normal_pass_workspace, m, n), B.device)
+
+    def __call__(self, p, q=None):
+        m, n = self.B.shape
+        if q is None:
+            q = torch.empty(n, dtype=F64, device=p.device)
+        check(lib().pamd_normal_pass(1 if self.B.dtype == torch.float32 else 0, m, n, _p(self.B),
+                                     self.B.stride(0), _p(p), _p(q), _p(self.ws), stream(p.device)),
+              "pamd_normal_pass")
+        return q
+
+
+class Residual:
+    """Compensated r = b - A x, g = A^T r and ||r||^2, one read of A."""
+
+    def __init__(self, A):
+        self.A = A if A.stride(1) == 1 else A.contiguous()
+        m, n = A.shape
+        self.ws = workspace(_query(lib().pamd_residual_workspace, m, n), A.device)
+
+    def __call__(self, x, b, want_r=False):
+        m, n = self.A.shape
+        dev = self.A.device
+        g = torch.empty(n, dtype=F64, device=dev)
+        rss = torch.empty(1, dtype=F64, device=dev)
+        r = torch.empty(m, dtype=F64, device=dev) if want_r else None
+        check(lib().pamd_residual_f64(m, n, _p(self.A), self.A.stride(0), _p(x.contiguous()), _p(b.contiguous()),
+                                      _p(r), _p(g), _p(rss), _p(self.ws), stream(dev)), "pamd_residual_f64")
+        return g, rss, r
+
+
+def trsm_rec_(X, R):
+    """In place X <- X R^{-1} (R upper), recursive MFMA TRSM for large m."""
+    m, n = X.shape
+    ws_bytes = _query(lib().pamd_gemm_workspace, m, n, n)
+    ws = workspace(ws_bytes, X.device)
+    check(lib().pamd_trsm_right_upper_rec_f64(m, n, _p(R), max(R.stride(0), n, 1), _p(X), max(X.stride(0), n, 1),
+                                              _p(ws), ws_bytes, stream(X.device)), "pamd_trsm_right_upper_rec_f64")
+    return X
+
+
 def sjlt_draw(n_short, n_long, k, seed, dev):
     rows = torch.empty((n_long, k), dtype=torch.int32, device=dev)
     vals = torch.empty((n_long, k), dtype=F64, device=dev)

@@ -1,0 +1,150 @@
+"""Mixed-precision kernels and the refined solver on the device, against the
+oracle's exact integer arithmetic, extended-precision residuals and the FP64
+least-squares solution."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-mixed] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def ops():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from parla_amd import ops as o
+    o.lib()
+    return o
+
+
+def _dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def test_gauss_q_matches_fill_quantised(ops):
+    _say("gauss_q vs gaussian_fill")
+    rows, m, seed = 37, 1000, 1234
+    Q = ops.gauss_q(rows, m, seed, torch.device("cuda"))
+    Z = ops.gaussian_fill(rows, m, 1.0, seed, torch.device("cuda"))
+    Qh = Q.float().cpu().numpy()
+    assert Qh.shape[1] == 1024 and np.all(Qh[:, m:] == 0)
+    np.testing.assert_array_equal(Qh[:, :m], orc.quantize_q(Z.cpu().numpy()))
+
+
+@pytest.mark.parametrize("m,n,with_b,P", [(700, 30, True, 4), (256, 64, False, 3), (1000, 65, True, 5),
+                                          (3, 1, False, 1)])
+def test_ozaki_split_bit_exact(ops, m, n, with_b, P):
+    _say(f"split m={m} n={n} b={with_b} P={P}")
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, n)) * np.exp(rng.uniform(-20, 20, size=(m, 1)))
+    b = rng.standard_normal(m) if with_b else None
+    sp = ops.OzakiSplit(_dev(A), None if b is None else _dev(b), P)
+    pieces, exps = orc.ozaki_split(A, b, P)
+    np.testing.assert_array_equal(sp.exps.cpu().numpy(), exps)
+    for p in range(P):
+        got = sp.piece(p).float().cpu().numpy()[:, :m].T
+        np.testing.assert_array_equal(got, pieces[p])
+
+
+@pytest.mark.parametrize("d,m,n,P", [(128, 512, 64, 4), (200, 700, 31, 4), (64, 256, 1, 2), (300, 1300, 130, 5)])
+def test_ozaki_sketch_exact(ops, d, m, n, P):
+    """The device product equals the int64 evaluation of Q @ pieces summed in
+    the same order: bit for bit."""
+    _say(f"ozaki sketch d={d} m={m} n={n} P={P}")
+    rng = np.random.default_rng(d + m)
+    A = rng.standard_normal((m, n))
+    b = rng.standard_normal(m)
+    Z = rng.standard_normal((d, m))
+    Q = ops.quantize_q(_dev(Z))
+    sp = ops.OzakiSplit(_dev(A), _dev(b), P)
+    scale = 2.0 ** -5 / np.sqrt(min(d, m))
+    out = ops.ozaki_sketch(Q, sp, scale).cpu().numpy()
+    pieces, exps = orc.ozaki_split(A, b, P)
+    ref = orc.ozaki_product(orc.quantize_q(Z), pieces, exps, scale)
+    np.testing.assert_array_equal(out, ref)
+    # and it is the sketch of [A | b] to the truncation level 2^-9P
+    full = scale * orc.quantize_q(Z) @ np.column_stack([A, b])
+    bound = scale * np.abs(orc.quantize_q(Z)).sum(axis=1, keepdims=True) * np.abs(np.column_stack([A, b])).max()
+    assert np.all(np.abs(out - full) <= 2.0 ** (-9 * P) * bound + 1e-13 * np.abs(full).max())
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_normal_pass(ops, dtype):
+    _say(f"normal pass {dtype}")
+    rng = np.random.default_rng(3)
+    for m, n in [(5000, 200), (777, 1024), (64, 3)]:
+        B = rng.standard_normal((m, n))
+        p = rng.standard_normal(n)
+        Bt = _dev(B).to(dtype)
+        q = ops.NormalOp(Bt)(_dev(p)).cpu().numpy()
+        Bh = Bt.double().cpu().numpy()
+        ref = Bh.T @ (Bh @ p)
+        np.testing.assert_allclose(q, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+
+
+def test_compensated_residual(ops):
+    """r and A^T r against correctly rounded values, with r 1e-10 of b."""
+    _say("compensated residual")
+    rng = np.random.default_rng(4)
+    m, n = 6000, 200
+    A = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, -6, n))
+    x = rng.standard_normal(n)
+    b = A @ x + 1e-10 * rng.standard_normal(m)
+    g, rss, r = ops.Residual(_dev(A))(_dev(x), _dev(b), want_r=True)
+    r_ref, g_ref, rss_ref = orc.residual_exact(A, x, b)
+    r = r.cpu().numpy()
+    np.testing.assert_allclose(r, r_ref, rtol=4 * np.finfo(float).eps, atol=0)
+    np.testing.assert_allclose(g.cpu().numpy(), g_ref, rtol=0, atol=1e-14 * np.abs(g_ref).max())
+    assert abs(rss.item() - rss_ref) <= 1e-12 * rss_ref
+    # the plain FP64 residual is far less accurate here
+    r64 = b - A @ x
+    assert np.max(np.abs(r64 - r_ref)) > 1e3 * np.max(np.abs(r - r_ref))
+
+
+@pytest.mark.parametrize("m,n", [(3000, 64), (5000, 200), (2000, 257), (777, 1024), (100, 33)])
+def test_trsm_recursive(ops, m, n):
+    _say(f"trsm m={m} n={n}")
+    rng = np.random.default_rng(m * 7 + n)
+    R = np.triu(rng.standard_normal((n, n))) + 4 * np.sqrt(n) * np.eye(n)
+    A = rng.standard_normal((m, n))
+    X = ops.trsm_rec_(_dev(A), _dev(R)).cpu().numpy()
+    ref = A @ np.linalg.inv(R)
+    import scipy.linalg as sla
+    ref = sla.solve_triangular(R, A.T, trans='T').T
+    np.testing.assert_allclose(X, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("cond", [1e4, 1e8])
+def test_spo_mixed_reaches_fp64_floor(ops, cond):
+    """The refined solution's backward error (exactly evaluated) is below
+    1e-12 or no worse than LAPACK's FP64 solution of the same problem, and x
+    agrees with it to the conditioning bound and with the oracle's
+    restatement of the algorithm on the same draw."""
+    import parla_amd as pa
+    from parla_amd import matmakers as mm
+    _say(f"SPOMixed cond={cond:g}")
+    rng = np.random.default_rng(11)
+    m, n = 6000, 120
+    U = mm.haar(m, n, rng)
+    V = mm.haar(n, n, rng)
+    s = np.logspace(0, -np.log10(cond), n)
+    A = (U * s) @ V.T
+    xs = rng.standard_normal(n)
+    b = A @ xs + 1e-8 * rng.standard_normal(m)
+    alg = pa.SPOMixed(4, npieces=4)
+    x, log = alg(_dev(A), _dev(b), 0.0, 1e-12, 100, 5)
+    x = x.cpu().numpy()
+    xl = np.linalg.lstsq(A, b, rcond=None)[0]
+    eta = orc.backward_error_extended(A, b, x)
+    eta_l = orc.backward_error_extended(A, b, xl)
+    assert eta <= max(1e-12, 1.5 * eta_l), (eta, eta_l, log.errors)
+    assert log.refinement_steps <= 6
+    assert np.linalg.norm(x - xl) <= 1e-15 * cond * 100 * np.linalg.norm(xl)
+    xo, eo = orc.spo_mixed(A, b, 1e-12, 5)
+    assert np.linalg.norm(x - xo) <= 1e-15 * cond * 100 * np.linalg.norm(xl)

@@ -336,3 +336,83 @@ def test_unpreconditioned_lsqr_trajectory_is_rounding_sensitive(golden):
     pertp = orc.lsqr(RoundedPc(), b, atol=1e-12, btol=1e-12, iter_lim=50)[7]
     k = min(refp.size, pertp.size)
     assert np.max(np.abs(pertp[:k] - refp[:k]) / refp[:k]) < 1e-6
+
+
+# --------------------------------------------------------------------------
+# mixed precision restatements
+# --------------------------------------------------------------------------
+
+def test_ozaki_split_reconstructs_within_truncation_bound():
+    rng = np.random.default_rng(21)
+    m, n = 900, 17
+    A = rng.standard_normal((m, n)) * np.exp(rng.uniform(-30, 30, size=(m, 1)))
+    b = rng.standard_normal(m)
+    M = np.column_stack([A, b])
+    for P in (1, 2, 3, 4, 5):
+        pieces, exps = orc.ozaki_split(A, b, P)
+        assert np.all(pieces == np.rint(pieces)) and np.abs(pieces).max() <= 256
+        rec = np.zeros_like(M)
+        for c in range(exps.shape[0]):
+            sl = slice(c * 256, (c + 1) * 256)
+            chunk_max = np.abs(M[sl]).max(axis=0)
+            assert np.all(chunk_max < np.ldexp(1.0, exps[c])) and np.all(chunk_max >= np.ldexp(0.5, exps[c]))
+            for p in range(P):
+                rec[sl] += np.ldexp(pieces[p, sl], (exps[c] - 8 - 9 * p)[None, :])
+            # remainder below 2^(e - 8 - 9P + 8) / 2 ... i.e. 2^-9P of the chunk scale
+            assert np.all(np.abs(rec[sl] - M[sl]) <= np.ldexp(1.0, exps[c] - 9 * P)[None, :])
+
+
+def test_ozaki_product_is_the_sketch_of_the_truncated_matrix():
+    """Exact integer sums: the product equals Q @ A_truncated computed with
+    Python integers (no floating point) up to the final FP64 rounding."""
+    from fractions import Fraction
+    rng = np.random.default_rng(22)
+    d, m, n, P = 3, 600, 2, 3
+    A = rng.standard_normal((m, n))
+    Q = orc.quantize_q(rng.standard_normal((d, m)))
+    pieces, exps = orc.ozaki_split(A, None, P)
+    out = orc.ozaki_product(Q, pieces, exps, 1.0)
+    for i in range(d):
+        for j in range(n):
+            tot = Fraction(0)
+            for k in range(m):
+                c = k // 256
+                at = sum(Fraction(int(pieces[p, k, j])) * Fraction(2) ** int(exps[c, j] - 8 - 9 * p) for p in range(P))
+                tot += int(Q[i, k]) * at
+            assert abs(float(tot) - out[i, j]) <= 4 * np.finfo(float).eps * abs(float(tot))
+
+
+def test_residual_exact_against_exact_rationals():
+    from fractions import Fraction
+    rng = np.random.default_rng(23)
+    A = rng.standard_normal((40, 5))
+    x = rng.standard_normal(5)
+    b = A @ x + 1e-12 * rng.standard_normal(40)
+    r, g, rss = orc.residual_exact(A, x, b)
+    rex = [Fraction(b[i]) - sum(Fraction(A[i, j]) * Fraction(x[j]) for j in range(5)) for i in range(40)]
+    assert [float(v) for v in rex] == list(r)  # correctly rounded
+    gex = [float(sum(Fraction(A[i, j]) * Fraction(r[i]) for i in range(40))) for j in range(5)]
+    assert gex == list(g)
+    re, ge, _ = orc.residual_extended(A, x, b)
+    np.testing.assert_allclose(re, r, rtol=1e-5)
+    # plain FP64 loses most digits of r at this residual level
+    assert np.max(np.abs((b - A @ x) - r)) > 1e-5 * np.max(np.abs(r))
+
+
+@pytest.mark.parametrize("cond", [1e3, 1e8])
+def test_spo_mixed_oracle_reaches_fp64_solution(cond):
+    rng = np.random.default_rng(24)
+    m, n = 3000, 40
+    U = np.linalg.qr(rng.standard_normal((m, n)))[0]
+    V = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    A = (U * np.logspace(0, -np.log10(cond), n)) @ V.T
+    xs = rng.standard_normal(n)
+    b = A @ xs + 1e-6 * rng.standard_normal(m)
+    x, etas = orc.spo_mixed(A, b, 1e-14, 3)
+    xl = np.linalg.lstsq(A, b, rcond=None)[0]
+    # eta below the target, or at the FP64 floor of the problem: no worse than
+    # LAPACK's own FP64 solution
+    eta = orc.backward_error_extended(A, b, x)
+    assert eta <= max(1e-14, orc.backward_error_extended(A, b, xl))
+    assert etas[0] > 1e-3 and len(etas) <= 6
+    assert np.linalg.norm(x - xl) <= 1e-15 * cond * 100 * np.linalg.norm(xl)
