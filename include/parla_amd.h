@@ -77,6 +77,13 @@ int pamd_gemv_workspace(int trans, int64_t m, int64_t n, int64_t* bytes);
  * drivers/least_squares.py:347,366. */
 int pamd_nrm2_f64(int64_t n, const double* x, double* result, void* stream);
 
+/* result[0] = ||x||_2 for long x (e.g. ||A||_F of a contiguous matrix): one
+ * partial sum of squares per workgroup (up to 1024, in `workspace`), then an
+ * ordered final sum.  Replaces np.linalg.norm of the whole matrix. */
+int pamd_nrm2_workspace(int64_t n, int64_t* bytes);
+int pamd_nrm2_large_f64(int64_t n, const double* x, double* result,
+                        void* workspace, void* stream);
+
 /* In-place triangular solve with the upper-triangular n x n R:
  * trans = 0: x <- R^{-1} x;  trans = 1: x <- R^{-T} x.
  * Replaces la.solve_triangular at drivers/least_squares.py:316,363 and

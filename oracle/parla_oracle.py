@@ -29,6 +29,9 @@ Parity status of each function (how it is pinned, see tests/test_oracle_*.py):
   against exact rational arithmetic; residual_extended against it.
 * spo_mixed -- the FP64 least-squares solution (scipy lstsq) and the target
   backward error on the reference's test families.
+* mlp_forward, mlp_loss_grad -- central finite differences of the loss;
+  bf16_round -- exhaustive comparison with torch's bfloat16 rounding;
+  leverage_scores -- sum = n, invariance under right scaling, closed forms.
 """
 import time
 import warnings
@@ -45,7 +48,9 @@ __all__ = [
     "a_lift_precond", "pcss2", "SketchAndPrecondLog", "spo", "orth",
     "lu_stabilize", "rs1", "rf1", "qb1", "svd1", "evd1", "loglinear_fit",
     "loglog_fit", "backward_error", "EPS", "quantize_q", "ozaki_split", "ozaki_product",
-    "residual_extended", "residual_exact", "backward_error_extended", "spo_mixed",
+    "residual_extended", "residual_exact", "backward_error_extended", "spo_mixed", "bf16_round",
+    "mlp_unpack", "mlp_effective", "mlp_forward", "mlp_loss_grad", "leverage_scores", "coherent_matrix",
+    "MLP_NPARAM",
 ]
 
 
@@ -724,6 +729,81 @@ def spo_mixed(A, b, tol, rng, sampling_factor=4, npieces=4, inner_tol=1e-6, max_
         x = x + la.solve_triangular(R, dz)
     x = best[1]
     return x, np.array(etas)
+
+
+# --------------------------------------------------------------------------
+# learned leverage-score sketch (no counterpart in the reference; the MLP is
+# pinned by finite differences, the labels by scipy's QR, the sampling
+# operator by the reference's sampling_operator semantics, sketching.py:204-232)
+# --------------------------------------------------------------------------
+
+MLP_IN, MLP_H = 16, 64
+MLP_SIZES = [("W1", (16, 64)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("w3", (64,)), ("b3", (1,))]
+MLP_NPARAM = sum(int(np.prod(sh)) for _, sh in MLP_SIZES)
+
+
+def bf16_round(x):
+    """Round float32 values to the nearest BF16 (ties to even), as float32."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def mlp_unpack(theta):
+    out, o = {}, 0
+    for name, sh in MLP_SIZES:
+        k = int(np.prod(sh))
+        out[name] = np.asarray(theta[o:o + k], dtype=np.float64).reshape(sh)
+        o += k
+    return out
+
+
+def mlp_effective(theta):
+    """Parameters as the device uses them: weights BF16-rounded, biases FP32."""
+    p = mlp_unpack(theta)
+    for k in ("W1", "W2", "w3"):
+        p[k] = bf16_round(p[k]).astype(np.float64)
+    for k in ("b1", "b2", "b3"):
+        p[k] = p[k].astype(np.float32).astype(np.float64)
+    return p
+
+
+def mlp_forward(p, X):
+    """y = w3 . relu(W2^T relu(W1^T asinh(x) + b1) + b2) + b3 for each row x of X."""
+    Xt = np.arcsinh(X)
+    h1 = np.maximum(Xt @ p["W1"] + p["b1"], 0.0)
+    h2 = np.maximum(h1 @ p["W2"] + p["b2"], 0.0)
+    return h2 @ p["w3"] + p["b3"][0], (h1, h2)
+
+
+def mlp_loss_grad(p, X, t, rows_total=None):
+    """(1/N) sum (y - t)^2 and its gradient, flat in the layout W1|b1|W2|b2|w3|b3."""
+    N = X.shape[0] if rows_total is None else rows_total
+    y, (h1, h2) = mlp_forward(p, X)
+    dy = 2.0 * (y - t) / N
+    dh2 = np.outer(dy, p["w3"]) * (h2 > 0)
+    dh1 = (dh2 @ p["W2"].T) * (h1 > 0)
+    g = {"W1": np.arcsinh(X).T @ dh1, "b1": dh1.sum(0), "W2": h1.T @ dh2, "b2": dh2.sum(0), "w3": h2.T @ dy,
+         "b3": np.array([dy.sum()])}
+    flat = np.concatenate([g[name].reshape(-1) for name, _ in MLP_SIZES])
+    return float(np.sum((y - t) ** 2) / N), flat
+
+
+def leverage_scores(A):
+    """Exact leverage scores ||Q[i, :]||^2 from scipy's FP64 Householder QR."""
+    Q = la.qr(np.asarray(A, dtype=np.float64), mode="economic")[0]
+    return np.sum(Q * Q, axis=1)
+
+
+def coherent_matrix(m, n, rng, heavy_frac=0.01, heavy_scale=100.0, cond=1e6):
+    """The learned-sketch test family: Gaussian rows, a fraction of rows scaled
+    by heavy_scale, right-scaled by diag(logspace(0, -log10 cond)) V^T."""
+    rng = np.random.default_rng(rng)
+    G = rng.standard_normal((m, n))
+    heavy = rng.choice(m, max(1, int(heavy_frac * m)), replace=False)
+    G[heavy] *= heavy_scale
+    V = la.qr(rng.standard_normal((n, n)))[0]
+    return (G * np.logspace(0, -np.log10(cond), n)) @ V.T, heavy
 
 
 # --------------------------------------------------------------------------

@@ -247,6 +247,26 @@ __global__ __launch_bounds__(1024) void nrm2_kernel(int64_t n, const double* __r
   if (threadIdx.x == 0) out[0] = sqrt(s);
 }
 
+// sum of squares: one partial per workgroup, then an ordered final sum
+__global__ __launch_bounds__(256) void sumsq_partial_kernel(int64_t n, const double* __restrict__ x,
+                                                            double* __restrict__ part) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += x[i] * x[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(1024) void sumsq_final_kernel(int64_t nparts, const double* __restrict__ part,
+                                                           double* __restrict__ out) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[0] = sqrt(s);
+}
+
 __global__ __launch_bounds__(1024) void trsv_kernel(int trans, int n, const double* __restrict__ R, int64_t ldr,
                                                     double* __restrict__ x) {
   extern __shared__ double smem[];
@@ -353,6 +373,29 @@ extern "C" int pamd_nrm2_f64(int64_t n, const double* x, double* result, void* s
   PAMD_ARG_CHECK(x != nullptr || n == 0, 2);
   PAMD_ARG_CHECK(result != nullptr, 3);
   nrm2_kernel<<<1, 1024, 0, (hipStream_t)stream>>>(n, x, result);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+constexpr int64_t NRM2_MAX_PARTS = 1024;
+
+extern "C" int pamd_nrm2_workspace(int64_t n, int64_t* bytes) {
+  PAMD_ARG_CHECK(n >= 0, 1);
+  PAMD_ARG_CHECK(bytes != nullptr, 2);
+  *bytes = NRM2_MAX_PARTS * (int64_t)sizeof(double);
+  return PAMD_OK;
+}
+
+extern "C" int pamd_nrm2_large_f64(int64_t n, const double* x, double* result, void* workspace, void* stream) {
+  PAMD_ARG_CHECK(n >= 0, 1);
+  PAMD_ARG_CHECK(x != nullptr || n == 0, 2);
+  PAMD_ARG_CHECK(result != nullptr, 3);
+  PAMD_ARG_CHECK(workspace != nullptr, 4);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t parts = imax64(1, imin64(NRM2_MAX_PARTS, pamd_cdiv(n, 256 * 16)));
+  sumsq_partial_kernel<<<(unsigned)parts, 256, 0, st>>>(n, x, (double*)workspace);
+  PAMD_LAUNCH_CHECK();
+  sumsq_final_kernel<<<1, 1024, 0, st>>>(parts, (const double*)workspace, result);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

@@ -2,7 +2,10 @@
 // kernels, its Adam step, and the row-sampling gather.
 //
 // Network: 16 -> 64 -> 64 -> 1, ReLU hidden layers, scalar output (the
-// predicted log of the normalised leverage score).  Weights are BF16 (the
+// predicted log of the normalised leverage score).  The inputs pass through
+// asinh first: pre-sketch features of coherent rows are orders of magnitude
+// larger than the rest, and asinh keeps their scale logarithmic, which is
+// the scale of the target.  Weights are BF16 (the
 // optimiser keeps FP32 masters and rounds them to nearest-even after each
 // step); every product and sum is FP32.  Parameter vector layout (FP32
 // master, and the same order for gradients):
@@ -94,7 +97,7 @@ __global__ __launch_bounds__(MLP_THREADS) void mlp_fwd_kernel(int64_t rows, cons
   if (r >= rows) return;
   float x[MLP_IN], h2[MLP_H];
 #pragma unroll
-  for (int k = 0; k < MLP_IN; ++k) x[k] = X[r * ldx + k];
+  for (int k = 0; k < MLP_IN; ++k) x[k] = asinhf(X[r * ldx + k]);
   y[r] = mlp_eval(s, x, hrow[threadIdx.x], h2);
 }
 
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(MLP_THREADS) void mlp_bwd_kernel(int64_t rows, doub
   const bool valid = r < rows;
   float x[MLP_IN], h2[MLP_H];
 #pragma unroll
-  for (int k = 0; k < MLP_IN; ++k) x[k] = valid ? X[r * ldx + k] : 0.f;
+  for (int k = 0; k < MLP_IN; ++k) x[k] = valid ? asinhf(X[r * ldx + k]) : 0.f;
   const float yv = mlp_eval(s, x, va[tid], h2);  // h1 -> va[tid]
   const float diff = valid ? yv - t[r] : 0.f;
   const float dy = (float)(2.0 * inv_total) * diff;

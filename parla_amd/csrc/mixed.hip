@@ -280,19 +280,36 @@ __global__ __launch_bounds__(NP_THREADS) void normal_pass_kernel(int64_t m, int6
     pr[q] = (c < n) ? p[c] : 0.0;
     acc[q] = 0.0;
   }
-  for (int64_t row = (int64_t)blockIdx.x * (NP_THREADS / 64) + wave; row < m; row += gw) {
-    double a[NPL];
+  int64_t row = (int64_t)blockIdx.x * (NP_THREADS / 64) + wave;
+  T a[NPL];
+  if (row < m) {
 #pragma unroll
     for (int q = 0; q < NPL; ++q) {
       const int64_t c = lane + 64 * q;
-      a[q] = (c < n) ? (double)B[row * ldb + c] : 0.0;
+      a[q] = (c < n) ? B[row * ldb + c] : T(0);
+    }
+  }
+  for (; row < m; row += gw) {
+    // the next row's loads are in flight while this row is reduced
+    const int64_t nrow = row + gw;
+    T an[NPL];
+    if (nrow < m) {
+#pragma unroll
+      for (int q = 0; q < NPL; ++q) {
+        const int64_t c = lane + 64 * q;
+        an[q] = (c < n) ? B[nrow * ldb + c] : T(0);
+      }
     }
     double s = 0.0;
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) s += a[q] * pr[q];
+    for (int q = 0; q < NPL; ++q) s += (double)a[q] * pr[q];
     s = wave_sum(s);
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) acc[q] += s * a[q];
+    for (int q = 0; q < NPL; ++q) acc[q] += s * (double)a[q];
+    if (nrow < m) {
+#pragma unroll
+      for (int q = 0; q < NPL; ++q) a[q] = an[q];
+    }
   }
 #pragma unroll
   for (int q = 0; q < NPL; ++q) red[wave][lane + 64 * q] = acc[q];

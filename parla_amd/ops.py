@@ -160,7 +160,12 @@ def nrm2(x):
     """||x||_2 as a 1-element device tensor."""
     x = x.contiguous()
     out = torch.empty(1, dtype=F64, device=x.device)
-    check(lib().pamd_nrm2_f64(x.numel(), _p(x), _p(out), stream(x.device)), "pamd_nrm2_f64")
+    n = x.numel()
+    if n > (1 << 18):  # long vectors: one partial per workgroup
+        ws = workspace(_query(lib().pamd_nrm2_workspace, n), x.device)
+        check(lib().pamd_nrm2_large_f64(n, _p(x), _p(out), _p(ws), stream(x.device)), "pamd_nrm2_large_f64")
+        return out
+    check(lib().pamd_nrm2_f64(n, _p(x), _p(out), stream(x.device)), "pamd_nrm2_f64")
     return out
 
 

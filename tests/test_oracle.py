@@ -416,3 +416,47 @@ def test_spo_mixed_oracle_reaches_fp64_solution(cond):
     assert eta <= max(1e-14, orc.backward_error_extended(A, b, xl))
     assert etas[0] > 1e-3 and len(etas) <= 6
     assert np.linalg.norm(x - xl) <= 1e-15 * cond * 100 * np.linalg.norm(xl)
+
+
+# --------------------------------------------------------------------------
+# learned sketch restatements
+# --------------------------------------------------------------------------
+
+def test_mlp_gradient_matches_finite_differences():
+    rng = np.random.default_rng(31)
+    theta = rng.standard_normal(orc.MLP_NPARAM) * 0.2
+    p = orc.mlp_unpack(theta)
+    X = rng.standard_normal((50, 16))
+    t = rng.standard_normal(50)
+    loss, g = orc.mlp_loss_grad(p, X, t)
+    h = 1e-6
+    idx = rng.choice(orc.MLP_NPARAM, 60, replace=False)
+    for i in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (orc.mlp_loss_grad(orc.mlp_unpack(tp), X, t)[0] - orc.mlp_loss_grad(orc.mlp_unpack(tm), X, t)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_bf16_round_matches_torch():
+    import torch
+    rng = np.random.default_rng(32)
+    x = np.concatenate([rng.standard_normal(10000).astype(np.float32) * 10.0 ** rng.integers(-20, 20, 10000),
+                        np.array([1.0, 1.00390625, 1.005859375, -2.5], dtype=np.float32)])
+    ref = torch.from_numpy(x).to(torch.bfloat16).float().numpy()
+    np.testing.assert_array_equal(orc.bf16_round(x), ref)
+
+
+def test_leverage_scores_invariants():
+    rng = np.random.default_rng(33)
+    A, heavy = orc.coherent_matrix(3000, 40, rng)
+    lev = orc.leverage_scores(A)
+    assert abs(lev.sum() - 40) < 1e-8 and lev.min() >= 0 and lev.max() <= 1 + 1e-12
+    # right scaling does not change leverage; heavy rows carry most of it
+    lev2 = orc.leverage_scores(A @ np.diag(np.logspace(0, 3, 40)))
+    np.testing.assert_allclose(lev, lev2, rtol=1e-6, atol=1e-12)
+    assert lev[heavy].sum() > 0.5 * 40
+    # closed form: orthonormal columns -> leverage = squared row norms
+    Q = la.qr(rng.standard_normal((200, 7)), mode="economic")[0]
+    np.testing.assert_allclose(orc.leverage_scores(Q), np.sum(Q ** 2, axis=1), rtol=1e-12)
