@@ -63,7 +63,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle timings")
     ap.add_argument("--ls-steps", type=int, default=10)
-    ap.add_argument("--legs", default="qb,ls,mixed", help="comma list of qb, ls, mixed")
+    ap.add_argument("--legs", default="qb,ls,mixed,learned", help="comma list of qb, ls, mixed, learned")
+    ap.add_argument("--train-matrices", type=int, default=2000)
     args = ap.parse_args()
     legs = set(args.legs.split(","))
     if args.gpus != 1:
@@ -187,6 +188,72 @@ def main():
         }
         del A3, b3
 
+    # ---------------------------------------------------------- learned LS
+    learned_extra = {}
+    if "learned" in legs:
+        import parla_amd.comps.sketchers.learned as lm
+        m5, n5, d5 = 131072, 512, 4096
+        log(f"C5: training set of {args.train_matrices} matrices {m5} x {n5} (labels from FP64 QR)")
+        t0 = time.perf_counter()
+        data = lm.training_set(args.train_matrices, m5, n5, 50000, dev, log=log)
+        torch.cuda.synchronize()
+        t_data = time.perf_counter() - t0
+        net = lm.LeverageMLP(seed=0)
+        t0 = time.perf_counter()
+        losses = net.fit(data, epochs=4, lr=2e-3, batch_rows=65536, log=log)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        rows_trained = sum(F.shape[0] for F, _ in data)
+        del data
+        A5, DG5 = lm.family_matrix(m5, n5, 777, dev)
+        lev5 = torch.exp(lm.leverage_labels(DG5).double()) * n5 / m5
+        del DG5
+        rng5 = np.random.default_rng(5)
+        b5 = A5 @ torch.from_numpy(rng5.standard_normal(n5)).to(dev) + \
+            1e-6 * torch.from_numpy(rng5.standard_normal(m5)).to(dev)
+        G5 = lm.presketch(n5, 4242, dev)
+        F5 = lm.features(A5, G5)
+        yl = net.predict(F5)
+        probs = {"learned": lm.sampling_probabilities(yl, 0.1, n5),
+                 "uniform": torch.full((m5,), 1.0 / m5, dtype=torch.float64, device=dev)}
+        rn5 = torch.sum(F5.double() ** 2, dim=1)
+        probs["rownorm"] = 0.9 * rn5 / rn5.sum() + 0.1 / m5
+        probe = lm.SPOLearned(net, d=d5, kappa_max=1e30)
+        kap, tv = {}, {}
+        for name, p5 in probs.items():
+            kap[name] = probe.sketch_factor(A5, b5, np.random.default_rng(2), probs=p5)[2]
+            tv[name] = float(0.5 * torch.sum(torch.abs(p5 - lev5 / n5)).item())
+        learned = lm.SPOLearned(net, d=d5, kappa_max=30.0)
+        plain5 = pa.SPO(pa.SkOpSJ(8, backend="device"), 4)
+
+        def learned_step(seed):
+            out["L5"] = learned(A5, b5, 0.0, 1e-12, 200, seed, logging=False)
+
+        def plain5_step(seed):
+            out["S5"] = plain5(A5, b5, 0.0, 1e-12, 200, seed, logging=False)
+
+        log("C5: learned-sketch SPO vs plain SJLT SPO")
+        t_l5 = timed_steps(learned_step, 3, 1, torch)
+        t_s5 = timed_steps(plain5_step, 3, 1, torch)
+        x_l5, lg5 = out["L5"]
+        x_s5 = out["S5"][0]
+        _, lgs5 = plain5(A5, b5, 0.0, 1e-12, 200, 7, logging=True)
+        learned_extra = {
+            "learned_workload": f"A {m5}x{n5} (Gaussian rows, 1% x100, right-scaled to cond 1e6), features A G16, "
+                                f"MLP 16-64-64-1 BF16 weights, row sampling d={d5}, SJLT fallback above kappa 30",
+            "learned_training": {"matrices": args.train_matrices, "rows": int(rows_trained),
+                                 "data_s": round(t_data, 2), "train_s": round(t_train, 2),
+                                 "epoch_losses": [round(v, 4) for v in losses]},
+            "learned_kappa_est": {k: round(v, 3) for k, v in kap.items()},
+            "learned_tv_to_leverage": {k: round(v, 4) for k, v in tv.items()},
+            "learned_fallback": bool(getattr(lg5, "fallback", False)),
+            "learned_ls_ms": round(t_l5 * 1e3, 3), "plain_sjlt_ls_ms": round(t_s5 * 1e3, 3),
+            "learned_lsqr_iters": int(lg5.errors.size - 1) if lg5.errors is not None else None,
+            "plain_sjlt_lsqr_iters": int(lgs5.errors.size - 1),
+            "learned_rel_diff_vs_plain_sjlt": float((torch.linalg.norm(x_l5 - x_s5) / torch.linalg.norm(x_s5)).item()),
+        }
+        del A5, b5, F5
+
     # ------------------------------------------------------------------ CPU
     cpu = None
     extra_cpu = {}
@@ -249,6 +316,7 @@ def main():
                            "device, LSQR tol 1e-12, logging off",
             **extra_cpu,
             **mixed_extra,
+            **learned_extra,
         },
     }
     print(json.dumps(line))

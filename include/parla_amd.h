@@ -354,6 +354,79 @@ int pamd_row_gather_f64(int64_t d, int64_t m, int64_t n, const int64_t* idx,
                         const double* scale, const double* A, int64_t lda,
                         double* out, int64_t ldo, void* stream);
 
+/* Batched potrf / trtri: problem i is the n[i] x n[i] matrix (ld n[i]) at
+ * element offset off[i] of the pool (n and off are device arrays; n[i] <= 512
+ * for potrf, nmax = max n[i]).  One workgroup per problem (per column for
+ * trtri), one launch for the whole batch; info[i] as pamd_potrf_upper_f64.
+ * The factorisations behind the per-problem CholeskyQR of a batch of
+ * sketches (drivers/least_squares.py:311, one problem per call there). */
+int pamd_potrf_batched_f64(int64_t nprob, const int64_t* n, const int64_t* off,
+                           double* pool, int* info, void* stream);
+int pamd_trtri_batched_f64(int64_t nprob, int64_t nmax, const int64_t* n,
+                           const int64_t* off, const double* R, double* Rinv,
+                           void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched least squares over problems of uneven size                        */
+/* ------------------------------------------------------------------------ */
+
+/* One problem of a batch (no counterpart in the reference, whose drivers
+ * solve one problem per call, drivers/least_squares.py:206).  A_i is FP32
+ * row-major with lda = n (n <= 512) at element offset a_off of the A pool;
+ * its m-vectors (b) start at mv_off of the m-vector pool, its n-vectors at
+ * nv_off of every n-vector pool, R_i (n x n FP64, ld n) at r_off; its
+ * 64-row blocks write partial vectors of 2n + 2 doubles from part_off of
+ * the partial pool; blk0 is the index of its first block. */
+typedef struct pamd_ls_desc {
+  int64_t m, n, a_off, mv_off, nv_off, r_off, blk0, part_off;
+} pamd_ls_desc;
+
+/* A 64-row block of problem `prob` starting at row `row0`. */
+typedef struct pamd_ls_block {
+  int32_t prob, row0;
+} pamd_ls_block;
+
+/* Row-block passes over all problems whose n has ceil(n/64) == npl
+ * (npl in 1, 2, 4, 8; blocks lists only those problems' blocks).  The launch
+ * shape (waves_per_wg in 1, 2, 4; blocks_per_wave 1..64) is the autotuned
+ * choice and does not change any result bit.
+ *   normal:   partial of A_i^T (A_i p_i) per block;
+ *   residual: partial of the compensated g_i = A_i^T (b_i - A_i x_i) and
+ *             ||r_i||^2 per block.  Replace LSQR's matvecs
+ *             (comps/determiter/lsqr.py:345-380) for a whole batch. */
+int pamd_bt_normal(int npl, const pamd_ls_desc* descs, const pamd_ls_block* blocks,
+                   int64_t nblk, int waves_per_wg, int blocks_per_wave,
+                   const float* A, const double* p, double* part, void* stream);
+int pamd_bt_residual(int npl, const pamd_ls_desc* descs, const pamd_ls_block* blocks,
+                     int64_t nblk, int waves_per_wg, int blocks_per_wave,
+                     const float* A, const double* b, const double* x,
+                     double* part, void* stream);
+
+/* Per-problem sum of the block partials in block order into out (n-vector
+ * pool); compensated = 1 also folds the compensation terms and writes
+ * ||r_i||^2 to scal[i]. */
+int pamd_bt_reduce(int64_t nprob, const pamd_ls_desc* descs, const double* part,
+                   int compensated, double* out, double* scal, void* stream);
+
+/* v_i <- R_i^{-1} v_i (trans 0) or R_i^{-T} v_i (trans 1) for every problem:
+ * the preconditioner solves of comps/preconditioning.py:28,37, batched. */
+int pamd_bt_trsv(int64_t nprob, const pamd_ls_desc* descs, int trans,
+                 const double* R, double* v, void* stream);
+
+/* Conjugate gradients on every problem's preconditioned normal equations,
+ * recurrences on the device; state[4 i] = r.r, state[4 i + 1] = 1 while
+ * problem i is active (0 freezes its vectors).  mode 0 starts from z = 0 with
+ * residual rhs, mode 1 takes one step given q = M p. */
+int pamd_bt_cg(int64_t nprob, const pamd_ls_desc* descs, int mode, double* state,
+               const double* rhs, double* r, double* p, const double* q,
+               double* z, void* stream);
+
+/* x_i += z_i for active problems; v_i = a_i for all problems. */
+int pamd_bt_axpy(int64_t nprob, const pamd_ls_desc* descs, const double* state,
+                 double* x, const double* z, void* stream);
+int pamd_bt_copy(int64_t nprob, const pamd_ls_desc* descs, const double* a,
+                 double* v, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,8 +40,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // LDS once instead of once per read.
 // Rows nb..31 of the staged block are zero and the diagonal block is padded
 // with the identity, so a last block narrower than 32 needs no special case.
-__global__ __launch_bounds__(POTRF_THREADS) void potrf_kernel(int64_t n, double* __restrict__ A, int64_t lda,
-                                                              int* __restrict__ info) {
+__device__ __forceinline__ void potrf_block(int64_t n, double* __restrict__ A, int64_t lda, int* __restrict__ info) {
   __shared__ double Rd[CB][CB + 1];  // the diagonal block; R11 row j replaces its row j at step j
   __shared__ double rinv[CB];        // 1 / R_jj
   __shared__ double Rs[CB][POTRF_MAX_N];
@@ -172,6 +171,20 @@ __global__ __launch_bounds__(POTRF_THREADS) void potrf_kernel(int64_t n, double*
   if (tid == 0) info[0] = bad;
 }
 
+__global__ __launch_bounds__(POTRF_THREADS) void potrf_kernel(int64_t n, double* __restrict__ A, int64_t lda,
+                                                              int* __restrict__ info) {
+  potrf_block(n, A, lda, info);
+}
+
+// problem i: the n[i] x n[i] matrix at pool + off[i] (ld n[i]); one workgroup each
+__global__ __launch_bounds__(POTRF_THREADS) void potrf_batched_kernel(const int64_t* __restrict__ nv,
+                                                                      const int64_t* __restrict__ off,
+                                                                      double* __restrict__ pool,
+                                                                      int* __restrict__ info) {
+  const int64_t n = nv[blockIdx.x];
+  potrf_block(n, pool + off[blockIdx.x], imax64(n, 1), info + blockIdx.x);
+}
+
 // Rinv = R^{-1} (upper; zeros below): workgroup j solves R x = e_j on rows 0..j.
 __global__ __launch_bounds__(256) void trtri_kernel(int64_t n, const double* __restrict__ R, int64_t ldr,
                                                     double* __restrict__ Ri, int64_t ldi) {
@@ -183,6 +196,24 @@ __global__ __launch_bounds__(256) void trtri_kernel(int64_t n, const double* __r
   __syncthreads();
   trsv_upper_block(0, sz, R, ldr, xs, smem + sz);
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Ri[i * ldi + j] = (i <= j) ? xs[i] : 0.0;
+}
+
+// batched R^{-1}: grid (max n, problems); column j of problem i when j < n[i]
+__global__ __launch_bounds__(256) void trtri_batched_kernel(const int64_t* __restrict__ nv,
+                                                            const int64_t* __restrict__ off,
+                                                            const double* __restrict__ Rp, double* __restrict__ Rip) {
+  extern __shared__ double smem[];
+  const int64_t n = nv[blockIdx.y];
+  const int64_t j = blockIdx.x;
+  if (j >= n) return;
+  const double* R = Rp + off[blockIdx.y];
+  double* Ri = Rip + off[blockIdx.y];
+  const int sz = (int)j + 1;
+  double* xs = smem;
+  for (int i = threadIdx.x; i < sz; i += blockDim.x) xs[i] = (i == sz - 1) ? 1.0 : 0.0;
+  __syncthreads();
+  trsv_upper_block(0, sz, R, n, xs, smem + sz);
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Ri[i * n + j] = (i <= j) ? xs[i] : 0.0;
 }
 
 // rows of B per workgroup: 16 lanes per row, tile of rows x n doubles <= 64 KiB
@@ -268,6 +299,35 @@ extern "C" int pamd_trtri_upper_f64(int64_t n, const double* R, int64_t ldr, dou
   if (n == 0) return PAMD_OK;
   const size_t shmem = (size_t)(n + TRSV_SCRATCH_DOUBLES) * sizeof(double);
   trtri_kernel<<<(unsigned)n, 256, shmem, (hipStream_t)stream>>>(n, R, ldr, Rinv, ldi);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_potrf_batched_f64(int64_t nprob, const int64_t* n, const int64_t* off, double* pool, int* info,
+                                      void* stream) {
+  PAMD_ARG_CHECK(nprob >= 0, 1);
+  PAMD_ARG_CHECK(n != nullptr || nprob == 0, 2);
+  PAMD_ARG_CHECK(off != nullptr || nprob == 0, 3);
+  PAMD_ARG_CHECK(pool != nullptr || nprob == 0, 4);
+  PAMD_ARG_CHECK(info != nullptr || nprob == 0, 5);
+  if (nprob == 0) return PAMD_OK;
+  potrf_batched_kernel<<<(unsigned)nprob, POTRF_THREADS, 0, (hipStream_t)stream>>>(n, off, pool, info);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_trtri_batched_f64(int64_t nprob, int64_t nmax, const int64_t* n, const int64_t* off,
+                                      const double* R, double* Rinv, void* stream) {
+  PAMD_ARG_CHECK(nprob >= 0, 1);
+  PAMD_ARG_CHECK(nmax >= 0 && nmax <= PAMD_TRSV_MAX_N, 2);
+  PAMD_ARG_CHECK(n != nullptr || nprob == 0, 3);
+  PAMD_ARG_CHECK(off != nullptr || nprob == 0, 4);
+  PAMD_ARG_CHECK(R != nullptr || nprob == 0, 5);
+  PAMD_ARG_CHECK(Rinv != nullptr || nprob == 0, 6);
+  if (nprob == 0 || nmax == 0) return PAMD_OK;
+  const size_t shmem = (size_t)(nmax + TRSV_SCRATCH_DOUBLES) * sizeof(double);
+  dim3 grid((unsigned)nmax, (unsigned)nprob);
+  trtri_batched_kernel<<<grid, 256, shmem, (hipStream_t)stream>>>(n, off, R, Rinv);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

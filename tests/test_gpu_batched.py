@@ -1,0 +1,150 @@
+"""Batched uneven least squares on the device: the batched row-block passes
+against numpy per problem, bit-invariance of results under every autotuner
+launch shape, graph replays equal to eager execution, and the solver against
+the FP64 least-squares solution of each problem."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[gpu-batched] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def bl():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from parla_amd import ops
+    ops.lib()
+    import parla_amd.drivers.batched_ls as bl_
+    return bl_
+
+
+def _problems(P, seed, mrange=(300, 3000), nrange=(5, 200), cond=(1e2, 1e8)):
+    rng = np.random.default_rng(seed)
+    As, bs, xs = [], [], []
+    for _ in range(P):
+        while True:
+            m = int(rng.integers(*mrange))
+            n = int(rng.integers(*nrange))
+            if m >= 8 * n:
+                break
+        c = 10 ** rng.uniform(np.log10(cond[0]), np.log10(cond[1]))
+        U = np.linalg.qr(rng.standard_normal((m, n)))[0]
+        V = np.linalg.qr(rng.standard_normal((n, n)))[0]
+        A = ((U * np.geomspace(1, 1 / c, n)) @ V.T).astype(np.float32)
+        x = rng.standard_normal(n)
+        b = A.astype(np.float64) @ x + 1e-6 * rng.standard_normal(m)
+        As.append(A)
+        bs.append(b)
+        xs.append(x)
+    return As, bs, xs
+
+
+def test_batched_passes_match_numpy_and_ignore_launch_shape(bl):
+    _say("batched passes")
+    from parla_amd import ops
+    As, bs, _ = _problems(13, 1, nrange=(1, 513), mrange=(600, 5000))
+    dev = torch.device("cuda")
+    B = bl.Batch(As, bs, dev)
+    rng = np.random.default_rng(2)
+    B.t.copy_(torch.from_numpy(rng.standard_normal(B.t.numel())).to(dev))
+    B.x.copy_(torch.from_numpy(rng.standard_normal(B.x.numel())).to(dev))
+    solver = bl.BatchedSPO()
+    outs = {}
+    for cand in bl.BatchTuner.CANDIDATES:
+        solver.tuner.cache = {}
+        solver.tuner.CANDIDATES = [cand]
+        solver.normal(B, B.q)
+        solver.residual(B)
+        outs[cand] = (B.q.clone(), B.g.clone(), B.rss.clone())
+    ref = outs[bl.BatchTuner.CANDIDATES[0]]
+    for cand, o in outs.items():
+        for a, b_ in zip(o, ref):
+            assert torch.equal(a, b_), cand  # bit-equal for every launch shape
+    q, g, rss = ref
+    for i, (A, b) in enumerate(zip(As, bs)):
+        Ad = A.astype(np.float64)
+        t = B.vec(B.t, i).cpu().numpy()
+        x = B.vec(B.x, i).cpu().numpy()
+        np.testing.assert_allclose(B.vec(q, i).cpu().numpy(), Ad.T @ (Ad @ t), rtol=1e-11,
+                                   atol=1e-11 * np.abs(Ad.T @ (Ad @ t)).max())
+        r_ex, g_ex, rss_ex = orc.residual_exact(Ad, x, b)
+        np.testing.assert_allclose(B.vec(g, i).cpu().numpy(), g_ex, rtol=0, atol=1e-13 * np.abs(g_ex).max())
+        assert abs(rss[i].item() - rss_ex) <= 1e-12 * rss_ex
+
+
+def test_batched_trsv_and_factors(bl):
+    _say("batched factor + trsv")
+    As, bs, _ = _problems(9, 3)
+    dev = torch.device("cuda")
+    B = bl.Batch(As, bs, dev)
+    solver = bl.BatchedSPO()
+    solver.factor(B, np.random.default_rng(4))
+    rng = np.random.default_rng(5)
+    v = torch.from_numpy(rng.standard_normal(B.t.numel())).to(dev)
+    B.t.copy_(v)
+    solver._trsv(B, B.t, False)
+    for i, A in enumerate(As):
+        R = B.R_view(i).cpu().numpy()
+        assert np.allclose(R, np.triu(R))
+        # R preconditions A: kappa(A R^-1) small
+        s = np.linalg.svd(A.astype(np.float64) @ np.linalg.inv(R), compute_uv=False)
+        assert s[0] / s[-1] < 10, (i, s[0] / s[-1])
+        np.testing.assert_allclose(R @ B.vec(B.t, i).cpu().numpy(), B.vec(v, i).cpu().numpy(), rtol=1e-8, atol=1e-9)
+
+
+def test_graph_replays_equal_eager(bl):
+    """A captured refinement step, replayed several times with eager work in
+    between, gives the eager step's result bit for bit each time."""
+    _say("graph replays")
+    As, bs, _ = _problems(7, 6)
+    dev = torch.device("cuda")
+    solver = bl.BatchedSPO(cg_iters=6)
+    B = bl.Batch(As, bs, dev)
+    solver.factor(B, np.random.default_rng(7))
+    solver.residual(B)
+    solver._copy(B, B.p, B.t)
+    solver.normal(B, B.q)
+    x0, g0, rss0, st0 = B.x.clone(), B.g.clone(), B.rss.clone(), B.state.clone()
+    solver.refine_step(B)
+    eager = (B.x.clone(), B.g.clone(), B.rss.clone())
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        solver.refine_step(B)
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    B.x.copy_(x0), B.g.copy_(g0), B.rss.copy_(rss0), B.state.copy_(st0)
+    with torch.cuda.graph(graph):
+        solver.refine_step(B)
+    for rep in range(3):
+        B.x.copy_(x0), B.g.copy_(g0), B.rss.copy_(rss0), B.state.copy_(st0)
+        torch.cuda.synchronize()
+        _ = torch.randn(1000, device=dev).sum().item()  # unrelated eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        for a, b_ in zip((B.x, B.g, B.rss), eager):
+            assert torch.equal(a, b_), rep
+
+
+def test_batched_spo_solves_every_problem(bl):
+    _say("batched solve")
+    As, bs, _ = _problems(24, 8, mrange=(2000, 8000), nrange=(32, 257), cond=(1e2, 1e10))
+    solver = bl.BatchedSPO(cg_iters=20)
+    xs, log = solver(As, bs, 1e-10, 9)
+    _say(f"refinement steps {log['refine_steps']}, final etas max {log['etas'][-1].max():.2e}")
+    for i, (A, b) in enumerate(zip(As, bs)):
+        Ad = A.astype(np.float64)
+        x = xs[i].cpu().numpy()
+        eta = orc.backward_error_extended(Ad, b, x)
+        xl = np.linalg.lstsq(Ad, b, rcond=None)[0]
+        eta_l = orc.backward_error_extended(Ad, b, xl)
+        assert eta <= max(1e-10, 2 * eta_l), (i, eta, eta_l)
+        s = np.linalg.svd(Ad, compute_uv=False)
+        assert np.linalg.norm(x - xl) <= 1e-15 * (s[0] / s[-1]) * 1e3 * np.linalg.norm(xl), i
