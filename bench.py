@@ -63,7 +63,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle timings")
     ap.add_argument("--ls-steps", type=int, default=10)
-    ap.add_argument("--legs", default="qb,ls,mixed,learned", help="comma list of qb, ls, mixed, learned")
+    ap.add_argument("--legs", default="qb,ls,mixed,learned,batched",
+                    help="comma list of qb, ls, mixed, learned, batched")
+    ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--train-matrices", type=int, default=2000)
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -254,6 +256,51 @@ def main():
         }
         del A5, b5, F5
 
+    # ---------------------------------------------------------- batched LS
+    batched_extra = {}
+    if "batched" in legs:
+        import oracle as orc
+        from parla_amd.drivers import batched_ls as bl
+        P4 = args.batch_problems
+        log(f"C4: generating {P4} uneven problems on the device")
+        As4, bs4 = bl.uneven_batch(P4, mm.data_seed(4), dev)
+        solver = bl.BatchedSPO(cg_iters=20)
+        solver(As4, bs4, 1e-10, 1)  # warm-up: tunes launch shapes, builds the graph once
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xs4, lg4 = solver(As4, bs4, 1e-10, 2)
+        torch.cuda.synchronize()
+        t_b4 = time.perf_counter() - t0
+        log("C4: plain FP64 SPO one problem at a time on a sample of 16 problems")
+        idx = np.random.default_rng(3).choice(P4, min(16, P4), replace=False)
+        plain4 = pa.SPO(pa.SkOpSJ(8, backend="device"), 4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in idx:
+            plain4(As4[i].double(), bs4[i], 0.0, 1e-12, 200, int(i), logging=False)
+        torch.cuda.synchronize()
+        t_p4 = (time.perf_counter() - t0) / len(idx)
+        etas = []
+        for i in idx[:6]:
+            etas.append(orc.backward_error_extended(As4[i].double().cpu().numpy(), bs4[i].cpu().numpy(),
+                                                    xs4[i].cpu().numpy()))
+        work = sum(A.shape[0] * A.shape[1] for A in As4)
+        batched_extra = {
+            "batched_workload": f"{P4} problems, m_i U[2000,50000], n_i U[32,512], m_i >= 8 n_i, cond log-U[1e2,1e10], "
+                                "FP32 A_i; SJLT d_i = 4 n_i, shifted CholeskyQR3, graph-replayed FP64 refinement "
+                                "(CG 20 per step) to eta <= 1e-10",
+            "batched_solves_per_s": round(P4 / t_b4, 2), "batched_ms_per_batch": round(t_b4 * 1e3, 2),
+            "batched_setup_ms": round(lg4["setup_s"] * 1e3, 2), "batched_solve_ms": round(lg4["solve_s"] * 1e3, 2),
+            "batched_refine_steps": int(lg4["refine_steps"]),
+            "batched_max_eta_device": float(lg4["etas"][-1].max()),
+            "batched_eta_extended_sample": [float(e) for e in etas],
+            "batched_sum_mn": int(work),
+            "plain_f64_one_at_a_time_solves_per_s": round(1.0 / t_p4, 2),
+            "batched_speedup_vs_plain_f64": round((P4 / t_b4) * t_p4, 2),
+            "batched_tuned_shapes": {f"{k[0]}/npl{k[1]}": list(v) for k, v in lg4["tuned"].items()},
+        }
+        del As4, bs4, xs4
+
     # ------------------------------------------------------------------ CPU
     cpu = None
     extra_cpu = {}
@@ -317,6 +364,7 @@ def main():
             **extra_cpu,
             **mixed_extra,
             **learned_extra,
+            **batched_extra,
         },
     }
     print(json.dumps(line))

@@ -139,10 +139,20 @@ def features(A, G):
     return (F / rms).to(torch.float32).contiguous()
 
 
-def leverage_labels(A):
-    """t_i = log(leverage_i * m / n) with leverage from the FP64 QR of A."""
+def leverage_labels(A, rounds=1):
+    """t_i = log(leverage_i * m / n), leverage_i = ||Q_i||^2 from the FP64
+    CholeskyQR of A: with kappa(A) <= 1e4 one round leaves Q orthonormal to
+    kappa^2 u <= 1e-8, so the labels are exact to that level; a failed pivot
+    falls back to the library's orth (CholeskyQR2 / Householder)."""
     m, n = A.shape
-    Q = ops.orth_q(A)
+    Q = A
+    info = torch.zeros(rounds, dtype=torch.int32, device=A.device)
+    for r in range(rounds):
+        G = ops.gemm(Q.t(), Q)
+        ops.potrf_(G, info[r:r + 1])
+        Q = ops.gemm(Q, ops.trtri(G))
+    if bool(info.any().item()):
+        Q = ops.orth_q(A)
     lev = torch.sum(Q * Q, dim=1)
     return torch.log(torch.clamp(lev * (m / n), min=1e-30)).to(torch.float32).contiguous()
 

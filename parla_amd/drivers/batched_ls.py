@@ -288,3 +288,30 @@ class BatchedSPO:
         log = {"setup_s": t_setup, "solve_s": t_solve, "refine_steps": steps, "etas": np.array(etas),
                "tuned": dict(self.tuner.cache)}
         return xs, log
+
+
+def uneven_batch(P, seed, dev, m_range=(2000, 50000), n_range=(32, 512), ratio=8, cond_range=(1e2, 1e10),
+                 noise=1e-8):
+    """The batched workload: P problems with m_i, n_i uniform (rejected until
+    m_i >= ratio n_i), cond_i log-uniform, A_i = U_i diag(s_i) V_i^T with Haar
+    factors (CholeskyQR2 of device Gaussians) and s_i geometric, stored FP32;
+    b_i = A_i x_i + noise N(0, 1) in FP64.  Returns (As, bs)."""
+    rng = np.random.default_rng(seed)
+    As, bs = [], []
+    for _ in range(P):
+        while True:
+            m = int(rng.integers(m_range[0], m_range[1] + 1))
+            n = int(rng.integers(n_range[0], n_range[1] + 1))
+            if m >= ratio * n:
+                break
+        cond = 10 ** rng.uniform(np.log10(cond_range[0]), np.log10(cond_range[1]))
+        U = ops.orth_q(ops.gaussian_fill(m, n, 1.0, int(rng.integers(0, 2 ** 63)), dev))
+        V = ops.orth_q(ops.gaussian_fill(n, n, 1.0, int(rng.integers(0, 2 ** 63)), dev))
+        s = torch.from_numpy(np.geomspace(1.0, 1.0 / cond, n)).to(dev)
+        A = ops.gemm(U * s, V.t()).to(torch.float32)
+        x = torch.from_numpy(rng.standard_normal(n)).to(dev)
+        b = A.double() @ x + noise * torch.from_numpy(rng.standard_normal(m)).to(dev)
+        As.append(A)
+        bs.append(b)
+    return As, bs
+
