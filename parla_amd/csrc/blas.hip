@@ -96,9 +96,27 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int64_t m0 = (int64_t)blockIdx.y * GBM;
-  const int64_t n0 = (int64_t)blockIdx.x * GBN;
-  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
+  // XCD-aware tile order: workgroup w runs on XCD w % 8, so tile t is given to
+  // the workgroup that keeps runs of consecutive tiles on one XCD and its L2.
+  // Consecutive tiles step along the dimension with fewer tiles: those tiles
+  // share the panel of the larger operand (a row panel of op(A) when N has
+  // fewer tiles, a column panel of op(B) when M has).
+  const uint32_t gx = gridDim.x, gy = gridDim.y;
+  const uint32_t tot = gx * gy * gridDim.z;
+  const uint32_t wid = (blockIdx.z * gy + blockIdx.y) * gx + blockIdx.x;
+  const uint32_t tile = xcd_tile(wid, tot);
+  uint32_t tx, ty;
+  const uint32_t tz = tile / (gx * gy), t2 = tile % (gx * gy);
+  if (gx <= gy) {
+    tx = t2 % gx;
+    ty = t2 / gx;
+  } else {
+    ty = t2 % gy;
+    tx = t2 / gy;
+  }
+  const int64_t m0 = (int64_t)ty * GBM;
+  const int64_t n0 = (int64_t)tx * GBN;
+  const int64_t kbeg = (int64_t)tz * kchunk;
   const int64_t kend = min(K, kbeg + kchunk);
 
   d4 acc[2][2];
@@ -147,7 +165,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
         if (row < M && col < N) {
           const double v = acc[i][j][r];
           if (part) {
-            part[((int64_t)blockIdx.z * M + row) * N + col] = v;
+            part[((int64_t)tz * M + row) * N + col] = v;
           } else {
             double* c = C + row * ldc + col;
             *c = (beta == 0.0) ? alpha * v : alpha * v + beta * (*c);

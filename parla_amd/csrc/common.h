@@ -67,6 +67,18 @@ __device__ __forceinline__ float dpp_f32(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
 }
 
+// MI355X dispatches workgroup w to XCD w % 8, each XCD with its own L2.
+// Returns the tile that workgroup w should process so that XCD j handles the
+// contiguous tile range [start_j, start_j + count_j): neighbouring tiles,
+// which share operand rows, then meet in the same L2.  A bijection on
+// [0, total) for any total.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t w, uint32_t total) {
+  constexpr uint32_t NXCD = 8;
+  const uint32_t q = total / NXCD, r = total % NXCD;
+  const uint32_t xcd = w % NXCD, local = w / NXCD;
+  return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
 // Returns x unchanged but opaque to the optimiser, at the point where it is
 // called: values derived from it are not hoisted out of the enclosing loop.
 // Used where loop-invariant lane masks would otherwise be precomputed for

@@ -271,17 +271,19 @@ def trsm_right_(B, R):
                                           stream(B.device)), "pamd_trsm_right_upper_f64")
 
 
-def cholqr2(Y):
+def cholqr2(Y, rounds=2):
     """Q spanning range(Y) by CholeskyQR2 (two rounds of G = Q^T Q, R = chol(G),
     Q <- Q R^{-1}); None when Y is too ill-conditioned for it (a pivot fails or
-    the first R's diagonal spread exceeds CHOLQR_MAX_COND)."""
+    the first R's diagonal spread exceeds CHOLQR_MAX_COND).  rounds=1 is plain
+    CholeskyQR: Q = Y R^{-1} spans range(Y) and equals the two-round Q times an
+    upper-triangular factor within kappa(Y)^2 u of the identity."""
     m, k = Y.shape
     if k == 0 or k > CHOLQR_MAX_K or m < k:
         return None
     Q = Y
-    info = torch.zeros(2, dtype=torch.int32, device=Y.device)
+    info = torch.zeros(rounds, dtype=torch.int32, device=Y.device)
     diag = None
-    for r in range(2):
+    for r in range(rounds):
         G = gemm(Q.t(), Q)
         potrf_(G, info[r:r + 1])
         if r == 0:
@@ -297,13 +299,14 @@ def cholqr2(Y):
     return Q
 
 
-def orth_q(Y):
+def orth_q(Y, rounds=2):
     """Orthonormal basis of range(Y) (m x min(m, n)).  CholeskyQR2 on MFMA when Y
     is tall and well conditioned, else Householder; the two agree up to the sign
-    of each column."""
+    of each column.  rounds=1: one CholeskyQR round (a well-conditioned basis of
+    range(Y), orthonormal to kappa(Y)^2 u)."""
     m, n = Y.shape
     if m >= n:
-        Q = cholqr2(Y)
+        Q = cholqr2(Y, rounds)
         if Q is not None:
             return Q
     return qr_q(Y)

@@ -21,6 +21,17 @@ def orth(S):
     return ops.like_input(ops.orth_q(St), ref)
 
 
+def orth_intermediate(S):
+    """A well-conditioned basis of range(S) for a stabilisation that is not the
+    last of a power iteration: one CholeskyQR round, S R^{-1} = Q T with T
+    upper triangular and within kappa(S)^2 u of the identity.  The next
+    product A (Q T) has the QR factor Q' of A Q up to that same T, so the
+    basis the iteration finally returns (stabilised by `orth`) is the one the
+    Householder stabiliser yields, up to column signs."""
+    St, ref = _dev(S)
+    return ops.like_input(ops.orth_q(St, rounds=1), ref)
+
+
 def lu_stabilize(S):
     """utils/linalg_wrappers.py:10-12: P @ L of LU with partial pivoting."""
     St, ref = _dev(S)
