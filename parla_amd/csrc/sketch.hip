@@ -64,7 +64,7 @@ __global__ void sjlt_draw_kernel(int64_t n_short, int64_t n_long, int k, uint64_
 // ---------------------------------------------------------------------------
 // CSC -> CSR (stable by column) in three passes over blocks of columns.
 // ---------------------------------------------------------------------------
-constexpr int CSR_CB = 256;  // columns per block
+constexpr int CSR_CB = 64;  // columns per block
 
 __global__ __launch_bounds__(256) void csr_count_kernel(int64_t n_short, int64_t n_long, int k,
                                                         const int32_t* __restrict__ rows, int32_t* __restrict__ cnt) {
@@ -119,34 +119,44 @@ __global__ __launch_bounds__(1024) void csr_scan_kernel(int64_t n_short, int nbl
 }
 
 // Thread t owns rows r with r % 256 == t and places them in column order.
-__global__ __launch_bounds__(256) void csr_fill_kernel(int64_t n_short, int64_t n_long, int k,
-                                                       const int32_t* __restrict__ rows,
-                                                       const double* __restrict__ vals,
-                                                       const int32_t* __restrict__ cnt, int32_t* __restrict__ cols,
-                                                       double* __restrict__ cvals) {
-  extern __shared__ int32_t lrows[];  // CSR_CB * k
+// One wavefront per block of CSR_CB columns walks its columns in order; the k
+// entries of a column go to their rows' next free slots together (the rows of
+// one column are distinct), so every row's entries land in column order, as
+// the sequential transpose would place them.  Columns drawn with replacement
+// (n_short < k) may repeat a row: their entries are placed one at a time.
+__global__ __launch_bounds__(64) void csr_fill_kernel(int64_t n_short, int64_t n_long, int k,
+                                                      const int32_t* __restrict__ rows,
+                                                      const double* __restrict__ vals,
+                                                      const int32_t* __restrict__ cnt, int32_t* __restrict__ cols,
+                                                      double* __restrict__ cvals) {
+  extern __shared__ int32_t cur[];  // n_short next-slot cursors
+  const int lane = threadIdx.x;
+  const int32_t* off = cnt + (int64_t)blockIdx.x * n_short;
+  for (int64_t r = lane; r < n_short; r += 64) cur[r] = off[r];
+  __syncthreads();
   const int64_t j0 = (int64_t)blockIdx.x * CSR_CB;
   const int64_t j1 = min(n_long, j0 + CSR_CB);
-  const int64_t ne = (j1 - j0) * k;
-  for (int64_t e = threadIdx.x; e < ne; e += blockDim.x) lrows[e] = rows[j0 * k + e];
-  __syncthreads();
-  const int32_t* off = cnt + (int64_t)blockIdx.x * n_short;
-  // per-thread running positions for up to 64 owned rows
-  for (int64_t rbase = threadIdx.x; rbase < n_short; rbase += 256 * 64) {
-    int32_t pos[64];
-    int nown = 0;
-    for (int64_t r = rbase; r < n_short && nown < 64; r += 256) pos[nown++] = off[r];
-    for (int64_t e = 0; e < ne; ++e) {
-      const int32_t r = lrows[e];
-      if ((r - rbase) >= 0 && ((r - rbase) % 256) == 0) {
-        const int q = (int)((r - rbase) / 256);
-        if (q < nown) {
-          const int32_t p = pos[q]++;
-          cols[p] = (int32_t)(j0 + e / k);
-          cvals[p] = vals[j0 * k + e];
-        }
+  const bool distinct = n_short >= k;
+  for (int64_t j = j0; j < j1; ++j) {
+    if (distinct) {
+      if (lane < k) {
+        const int64_t e = j * k + lane;
+        const int32_t r = rows[e];
+        const int32_t p = cur[r];
+        cur[r] = p + 1;
+        cols[p] = (int32_t)j;
+        cvals[p] = vals[e];
+      }
+    } else if (lane == 0) {
+      for (int l = 0; l < k; ++l) {
+        const int64_t e = j * k + l;
+        const int32_t r = rows[e];
+        const int32_t p = cur[r]++;
+        cols[p] = (int32_t)j;
+        cvals[p] = vals[e];
       }
     }
+    PAMD_WAVE_SYNC();  // this column's cursor updates before the next column reads them
   }
 }
 
@@ -313,8 +323,7 @@ extern "C" int pamd_sjlt_csr(int64_t n_short, int64_t n_long, int k, const int32
   csr_scan_kernel<<<1, 1024, 0, st>>>(n_short, nblk, cnt, rowptr);
   PAMD_LAUNCH_CHECK();
   if (n_long > 0) {
-    csr_fill_kernel<<<nblk, 256, CSR_CB * k * sizeof(int32_t), st>>>(n_short, n_long, k, rows, vals, cnt, cols,
-                                                                    cvals);
+    csr_fill_kernel<<<nblk, 64, n_short * sizeof(int32_t), st>>>(n_short, n_long, k, rows, vals, cnt, cols, cvals);
     PAMD_LAUNCH_CHECK();
   }
   return PAMD_OK;

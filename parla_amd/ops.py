@@ -271,31 +271,67 @@ def trsm_right_(B, R):
                                           stream(B.device)), "pamd_trsm_right_upper_f64")
 
 
+NEWTON_MAX_E = 1e-4  # second round by a Newton correction while ||Q1^T Q1 - I||_F stays below this
+
+
+def _chol_round(Q, info):
+    G = gemm(Q.t(), Q)
+    potrf_(G, info)
+    return G, gemm(Q, trtri(G))  # Q R^{-1} as an MFMA GEMM
+
+
+def _newton_round(Q):
+    """Second CholeskyQR round for a nearly orthonormal Q (Q^T Q = I + E):
+    the Cholesky factor of I + E is I + X with X + X^T = E - X^T X, solved by
+    two fixed-point steps X = H(E - X^T X) (H takes the upper triangle and
+    halves the diagonal; the error after them is O(||E||^3)), and
+    (I + X)^{-1} = I - X + X^2 + O(||X||^3).  Three k x k products and one
+    m x k x k GEMM instead of potrf and trtri.  Returns (Q (I + X)^{-1}, ||E||_F)."""
+    k = Q.shape[1]
+    E = gemm(Q.t(), Q)
+    E.diagonal().sub_(1.0)
+
+    def H(F):
+        Xh = torch.triu(F)
+        Xh.diagonal().mul_(0.5)
+        return Xh
+
+    X = H(E)
+    X = H(E - gemm(X.t(), X))
+    M = gemm(X, X)
+    M -= X
+    M.diagonal().add_(1.0)
+    return gemm(Q, M), torch.linalg.matrix_norm(E)
+
+
 def cholqr2(Y, rounds=2):
     """Q spanning range(Y) by CholeskyQR2 (two rounds of G = Q^T Q, R = chol(G),
     Q <- Q R^{-1}); None when Y is too ill-conditioned for it (a pivot fails or
-    the first R's diagonal spread exceeds CHOLQR_MAX_COND).  rounds=1 is plain
-    CholeskyQR: Q = Y R^{-1} spans range(Y) and equals the two-round Q times an
-    upper-triangular factor within kappa(Y)^2 u of the identity."""
+    the first R's diagonal spread exceeds CHOLQR_MAX_COND).  The second round,
+    whose Gram matrix is I + E with E of order kappa(Y)^2 u, is a Newton
+    correction (_newton_round) while ||E||_F <= NEWTON_MAX_E, else a second
+    potrf round.  rounds=1 is plain CholeskyQR: Q = Y R^{-1} spans range(Y)
+    and equals the two-round Q times an upper-triangular factor within
+    kappa(Y)^2 u of the identity."""
     m, k = Y.shape
     if k == 0 or k > CHOLQR_MAX_K or m < k:
         return None
-    Q = Y
-    info = torch.zeros(rounds, dtype=torch.int32, device=Y.device)
-    diag = None
-    for r in range(rounds):
-        G = gemm(Q.t(), Q)
-        potrf_(G, info[r:r + 1])
-        if r == 0:
-            diag = torch.diagonal(G).clone()
-        Q = gemm(Q, trtri(G))  # Y R^{-1} as an MFMA GEMM
-    # one synchronisation: pivots and the conditioning estimate
-    flags = info.cpu().numpy()
-    if flags.any():
+    info = torch.zeros(2, dtype=torch.int32, device=Y.device)
+    G, Q1 = _chol_round(Y, info[0:1])
+    diag = torch.diagonal(G)
+    if rounds == 1:
+        Q, enorm = Q1, torch.zeros((), dtype=F64, device=Y.device)
+    else:
+        Q, enorm = _newton_round(Q1)
+    # one synchronisation: pivot, conditioning estimate, size of E
+    dd = torch.cat([diag.abs().min().reshape(1), diag.abs().max().reshape(1), enorm.reshape(1),
+                    info[0:1].to(F64)]).cpu().numpy()
+    if dd[3] != 0 or not np.all(np.isfinite(dd)) or dd[0] <= 0 or dd[1] / dd[0] > CHOLQR_MAX_COND:
         return None
-    d = np.abs(diag.cpu().numpy())
-    if not np.all(np.isfinite(d)) or d.min() <= 0 or d.max() / d.min() > CHOLQR_MAX_COND:
-        return None
+    if rounds == 2 and dd[2] > NEWTON_MAX_E:
+        _, Q = _chol_round(Q1, info[1:2])
+        if int(info[1].item()) != 0:
+            return None
     return Q
 
 
