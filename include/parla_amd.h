@@ -199,6 +199,17 @@ int pamd_trtri_upper_f64(int64_t n, const double* R, int64_t ldr, double* Rinv,
 
 /* In place B (m x n) <- B R^{-1} with R upper triangular n x n (n <= 2048):
  * the Q = Y R^{-1} step of CholeskyQR. */
+/* X = Y R^{-1} (R upper k x k, k <= 512; X may alias Y) on FP64 MFMA: one
+ * workgroup per 16-row strip held in LDS, 16-column blocks solved left to
+ * right with the inverses of R's 16 x 16 diagonal blocks (workspace from
+ * pamd_trsm_mfma_workspace).  The Y R^{-1} step of CholeskyQR (orth,
+ * utils/linalg_wrappers.py:6-7) in one launch, without forming R^{-1}. */
+int pamd_trsm_mfma_workspace(int64_t k, int64_t* bytes);
+int pamd_trsm_right_upper_mfma_f64(int64_t m, int64_t k, const double* R,
+                                   int64_t ldr, const double* Y, int64_t ldy,
+                                   double* X, int64_t ldx, void* workspace,
+                                   void* stream);
+
 int pamd_trsm_right_upper_f64(int64_t m, int64_t n, const double* R,
                               int64_t ldr, double* B, int64_t ldb, void* stream);
 

@@ -150,7 +150,7 @@ def leverage_labels(A, rounds=1):
     for r in range(rounds):
         G = ops.gemm(Q.t(), Q)
         ops.potrf_(G, info[r:r + 1])
-        Q = ops.gemm(Q, ops.trtri(G))
+        Q = ops.trsm_mfma(Q, G)
     if bool(info.any().item()):
         Q = ops.orth_q(A)
     lev = torch.sum(Q * Q, dim=1)

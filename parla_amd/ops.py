@@ -274,10 +274,23 @@ def trsm_right_(B, R):
 NEWTON_MAX_E = 1e-4  # second round by a Newton correction while ||Q1^T Q1 - I||_F stays below this
 
 
+def trsm_mfma(Y, R, out=None):
+    """Y R^{-1} (R upper, k <= 512; only R's upper triangle is read) by the
+    MFMA strip solver; out may be Y itself."""
+    m, k = Y.shape
+    Yc = Y if Y.stride(1) == 1 else Y.contiguous()
+    X = torch.empty((m, k), dtype=F64, device=Y.device) if out is None else out
+    ws = workspace(_query(lib().pamd_trsm_mfma_workspace, k), Y.device)
+    check(lib().pamd_trsm_right_upper_mfma_f64(m, k, _p(R), max(R.stride(0), k, 1), _p(Yc), max(Yc.stride(0), k, 1),
+                                               _p(X), max(X.stride(0), k, 1), _p(ws), stream(Y.device)),
+          "pamd_trsm_right_upper_mfma_f64")
+    return X
+
+
 def _chol_round(Q, info):
     G = gemm(Q.t(), Q)
     potrf_(G, info)
-    return G, gemm(Q, trtri(G))  # Q R^{-1} as an MFMA GEMM
+    return G, trsm_mfma(Q, G)  # Q R^{-1}
 
 
 def _newton_round(Q):

@@ -329,3 +329,21 @@ def test_trtri_matches_scipy(ops, n):
     Ri = host(ops.trtri(dev(R)))
     assert np.all(np.tril(Ri, -1) == 0)
     assert la.norm(R @ Ri - np.eye(n)) <= 100 * n * EPS * la.norm(R) * la.norm(Ri)
+
+
+@pytest.mark.parametrize("m,k", [(8192, 266), (100, 7), (3000, 300), (777, 512), (16, 16), (33, 1)])
+def test_trsm_mfma_matches_scipy(ops, m, k):
+    """X = Y R^{-1} by the MFMA strip solver, out of place and in place; only
+    the upper triangle of R is read (junk below it)."""
+    import scipy.linalg as sla
+    rng = np.random.default_rng(m + k)
+    R = np.triu(rng.standard_normal((k, k))) + 2 * np.sqrt(k) * np.eye(k)
+    Rj = R + np.tril(rng.standard_normal((k, k)), -1)
+    Y = rng.standard_normal((m, k))
+    ref = sla.solve_triangular(R, Y.T, trans='T').T
+    Yd = torch.from_numpy(Y).cuda()
+    X = ops.trsm_mfma(Yd, torch.from_numpy(Rj).cuda()).cpu().numpy()
+    np.testing.assert_allclose(X, ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+    ops.trsm_mfma(Yd, torch.from_numpy(Rj).cuda(), out=Yd)
+    np.testing.assert_allclose(Yd.cpu().numpy(), ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+
