@@ -139,11 +139,11 @@ def features(A, G):
     return (F / rms).to(torch.float32).contiguous()
 
 
-def leverage_labels(A, rounds=1):
+def leverage_labels(A, rounds=2):
     """t_i = log(leverage_i * m / n), leverage_i = ||Q_i||^2 from the FP64
-    CholeskyQR of A: with kappa(A) <= 1e4 one round leaves Q orthonormal to
-    kappa^2 u <= 1e-8, so the labels are exact to that level; a failed pivot
-    falls back to the library's orth (CholeskyQR2 / Householder)."""
+    CholeskyQR of A (`rounds` rounds: after one round Q is orthonormal to
+    kappa(A)^2 u, after two to u while kappa(A) << 1e8); a failed pivot falls
+    back to the library's orth (CholeskyQR2 / Householder)."""
     m, n = A.shape
     Q = A
     info = torch.zeros(rounds, dtype=torch.int32, device=A.device)
@@ -176,7 +176,7 @@ def training_set(num, m, n, seed, dev, log=None):
     for i in range(num):
         A, DG = family_matrix(m, n, seed + i, dev)
         Gp = presketch(n, seed + 10_000_000 + i, dev)
-        data.append((features(A, Gp), leverage_labels(DG)))
+        data.append((features(A, Gp), leverage_labels(DG, rounds=1)))  # kappa(D G) < 10: one round is exact
         del A, DG
         if log and (i + 1) % max(1, num // 10) == 0:
             log(f"training set: {i + 1}/{num} matrices")

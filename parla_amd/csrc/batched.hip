@@ -8,7 +8,7 @@
 //   * row-block passes (normal operator A^T A p, compensated residual): the
 //     unit of work is a fixed 64-row block of one problem, handled by one
 //     wavefront; each block writes its own partial vector and a second kernel
-//     sums a problem's blocks in index order.  The summation order therefore
+//     sums a problem's blocks in a fixed order.  The summation order therefore
 //     does not depend on how blocks are assigned to workgroups, which is the
 //     only thing the autotuned launch shape changes: results are bit-equal
 //     for every tile choice.  Blocks are grouped by NPL = ceil(n_i / 64) so a
@@ -66,19 +66,32 @@ __global__ __launch_bounds__(256) void bt_normal_kernel(const Desc* __restrict__
       acc[q] = 0.0;
     }
     const int64_t r1 = imin64(d.m, (int64_t)bk.row0 + BT_BLOCK_ROWS);
-    for (int64_t row = bk.row0; row < r1; ++row) {
-      double a[NPL];
+    float a[NPL];
 #pragma unroll
-      for (int q = 0; q < NPL; ++q) {
-        const int64_t c = lane + 64 * q;
-        a[q] = (c < n) ? (double)Ai[row * n + c] : 0.0;
+    for (int q = 0; q < NPL; ++q) {
+      const int64_t c = lane + 64 * q;
+      a[q] = (c < n) ? Ai[(int64_t)bk.row0 * n + c] : 0.f;
+    }
+    for (int64_t row = bk.row0; row < r1; ++row) {
+      // the next row's loads are in flight while this row is reduced
+      float an[NPL];
+      if (row + 1 < r1) {
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) {
+          const int64_t c = lane + 64 * q;
+          an[q] = (c < n) ? Ai[(row + 1) * n + c] : 0.f;
+        }
       }
       double s = 0.0;
 #pragma unroll
-      for (int q = 0; q < NPL; ++q) s += a[q] * pr[q];
+      for (int q = 0; q < NPL; ++q) s += (double)a[q] * pr[q];
       s = wave_sum(s);
 #pragma unroll
-      for (int q = 0; q < NPL; ++q) acc[q] += s * a[q];
+      for (int q = 0; q < NPL; ++q) acc[q] += s * (double)a[q];
+      if (row + 1 < r1) {
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) a[q] = an[q];
+      }
     }
     double* pw = part + d.part_off + (int64_t)(bk.row0 / BT_BLOCK_ROWS) * (2 * n + 2);
 #pragma unroll
@@ -135,12 +148,22 @@ __global__ __launch_bounds__(256) void bt_residual_kernel(const Desc* __restrict
     }
     double rr = 0.0;
     const int64_t r1 = imin64(d.m, (int64_t)bk.row0 + BT_BLOCK_ROWS);
+    float af[NPL];
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      const int64_t c = lane + 64 * q;
+      af[q] = (c < n) ? Ai[(int64_t)bk.row0 * n + c] : 0.f;
+    }
     for (int64_t row = bk.row0; row < r1; ++row) {
       double a[NPL];
 #pragma unroll
-      for (int q = 0; q < NPL; ++q) {
-        const int64_t c = lane + 64 * q;
-        a[q] = (c < n) ? (double)Ai[row * n + c] : 0.0;
+      for (int q = 0; q < NPL; ++q) a[q] = (double)af[q];
+      if (row + 1 < r1) {  // prefetch the next row
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) {
+          const int64_t c = lane + 64 * q;
+          af[q] = (c < n) ? Ai[(row + 1) * n + c] : 0.f;
+        }
       }
       double s = 0.0, cs = 0.0;
 #pragma unroll
@@ -186,34 +209,60 @@ __global__ __launch_bounds__(256) void bt_residual_kernel(const Desc* __restrict
   }
 }
 
-// sum a problem's block partials in block order: out (n-vector) and, for the
-// residual, the compensation terms and ||r||^2 (into scal[prob])
-__global__ __launch_bounds__(256) void bt_reduce_kernel(const Desc* __restrict__ D, const double* __restrict__ part,
-                                                        int compensated, double* __restrict__ out,
-                                                        double* __restrict__ scal) {
+// sum a problem's block partials in a fixed order: one workgroup per problem,
+// 64-column chunks, its 16 wavefronts take the blocks bb = w, w + 16, ... and
+// their sums are combined in wavefront order; out (n-vector pool) and, for
+// the residual, the compensation terms folded in and ||r_i||^2 in scal[prob]
+constexpr int BT_RED_WAVES = 16;
+
+__global__ __launch_bounds__(64 * BT_RED_WAVES) void bt_reduce_kernel(const Desc* __restrict__ D,
+                                                                      const double* __restrict__ part,
+                                                                      int compensated, double* __restrict__ out,
+                                                                      double* __restrict__ scal) {
+  __shared__ double rs[BT_RED_WAVES][64];
+  __shared__ double rc[BT_RED_WAVES][64];
   const Desc d = D[blockIdx.x];
   const int64_t n = d.n;
   const int64_t nb = pamd_cdiv(d.m, BT_BLOCK_ROWS);
   const double* p0 = part + d.part_off;
   const int64_t ps = 2 * n + 2;
-  for (int64_t c = threadIdx.x; c < n; c += blockDim.x) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t cb = 0; cb <= n; cb += 64) {  // the chunk holding column n carries ||r||^2
+    const int64_t c = cb + lane;
     double s = 0.0, cc = 0.0;
-    for (int64_t bb = 0; bb < nb; ++bb) {
-      if (compensated) {
-        double t, e;
-        bt_two_sum(s, p0[bb * ps + c], t, e);
-        s = t;
-        cc = bt_add(cc, bt_add(p0[bb * ps + n + c], e));
-      } else {
-        s += p0[bb * ps + c];
+    if (c < n) {
+      for (int64_t bb = wave; bb < nb; bb += BT_RED_WAVES) {
+        if (compensated) {
+          double t, e;
+          bt_two_sum(s, p0[bb * ps + c], t, e);
+          s = t;
+          cc = bt_add(cc, bt_add(p0[bb * ps + n + c], e));
+        } else {
+          s += p0[bb * ps + c];
+        }
       }
+    } else if (c == n && compensated) {
+      for (int64_t bb = wave; bb < nb; bb += BT_RED_WAVES) s += p0[bb * ps + 2 * n];
     }
-    out[d.nv_off + c] = compensated ? bt_add(s, cc) : s;
-  }
-  if (compensated && threadIdx.x == 0) {
-    double t = 0.0;
-    for (int64_t bb = 0; bb < nb; ++bb) t += p0[bb * ps + 2 * n];
-    scal[blockIdx.x] = t;
+    rs[wave][lane] = s;
+    rc[wave][lane] = cc;
+    __syncthreads();
+    if (wave == 0) {
+      double S = 0.0, C = 0.0;
+      for (int w = 0; w < BT_RED_WAVES; ++w) {
+        if (compensated && c < n) {
+          double t, e;
+          bt_two_sum(S, rs[w][lane], t, e);
+          S = t;
+          C = bt_add(C, bt_add(rc[w][lane], e));
+        } else {
+          S += rs[w][lane];
+        }
+      }
+      if (c < n) out[d.nv_off + c] = compensated ? bt_add(S, C) : S;
+      else if (c == n && compensated) scal[blockIdx.x] = S;
+    }
+    __syncthreads();
   }
 }
 
@@ -364,7 +413,8 @@ extern "C" int pamd_bt_reduce(int64_t nprob, const pamd_ls_desc* descs, const do
   PAMD_ARG_CHECK(out != nullptr || nprob == 0, 5);
   PAMD_ARG_CHECK(scal != nullptr || !compensated || nprob == 0, 6);
   if (nprob == 0) return PAMD_OK;
-  bt_reduce_kernel<<<(unsigned)nprob, 256, 0, (hipStream_t)stream>>>((const Desc*)descs, part, compensated, out, scal);
+  bt_reduce_kernel<<<(unsigned)nprob, 64 * BT_RED_WAVES, 0, (hipStream_t)stream>>>((const Desc*)descs, part,
+                                                                                  compensated, out, scal);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

@@ -49,20 +49,33 @@ __device__ __forceinline__ uint16_t bf16_bits_of_int(double a) {
 // same Philox counter as the FP64 fill of a rows x m matrix, so the FP64 fill
 // times 32, rounded to the nearest integer, is this operator
 // ---------------------------------------------------------------------------
-__global__ void gauss_q_kernel(int64_t rows, int64_t m, int64_t kp, uint64_t seed, uint64_t offset,
-                               uint16_t* __restrict__ Q, int64_t ldq) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= rows * kp) return;
-  const int64_t i = idx / kp, k = idx % kp;
-  uint16_t v = 0;
+__device__ __forceinline__ uint16_t gauss_q_value(double z) {
+  return bf16_bits_of_int(fmin(256.0, fmax(-256.0, rint(32.0 * z))));
+}
+
+// grid (ceil(kp / 512), rows): thread t of block x covers columns k = 2 (256 x + t)
+// and k + 1 of row blockIdx.y; both normals of one Philox pair serve the two
+// columns when the pair is aligned with them (one generator call per two
+// entries), two calls otherwise.
+__global__ __launch_bounds__(256) void gauss_q_kernel(int64_t m, int64_t kp, uint64_t seed, uint64_t offset,
+                                                      uint16_t* __restrict__ Q, int64_t ldq) {
+  const int64_t i = blockIdx.y;
+  const int64_t k = 2 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= kp) return;
+  uint16_t v0 = 0, v1 = 0;
+  const uint64_t e0 = offset + (uint64_t)(i * m + k);
   if (k < m) {
-    const uint64_t e = offset + (uint64_t)(i * m + k);
     double z0, z1;
-    normal_pair(seed, e >> 1, z0, z1);
-    const double q = fmin(256.0, fmax(-256.0, rint(32.0 * ((e & 1) ? z1 : z0))));
-    v = bf16_bits_of_int(q);
+    normal_pair(seed, e0 >> 1, z0, z1);
+    v0 = gauss_q_value((e0 & 1) ? z1 : z0);
+    if (k + 1 < m) {
+      const uint64_t e1 = e0 + 1;
+      if ((e1 >> 1) != (e0 >> 1)) normal_pair(seed, e1 >> 1, z0, z1);
+      v1 = gauss_q_value((e1 & 1) ? z1 : z0);
+    }
   }
-  Q[i * ldq + k] = v;
+  Q[i * ldq + k] = v0;
+  if (k + 1 < kp) Q[i * ldq + k + 1] = v1;
 }
 
 // ---------------------------------------------------------------------------
@@ -322,13 +335,30 @@ __global__ __launch_bounds__(NP_THREADS) void normal_pass_kernel(int64_t m, int6
   }
 }
 
-__global__ void reduce_partials_kernel(int64_t n, int64_t nparts, const double* __restrict__ part,
-                                       double* __restrict__ out) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+// out[c] = sum_g part[g][c] in a fixed order: workgroup per 64 columns, its 16
+// wavefronts take the partials g = w, w + 16, ... (coalesced 64-column rows),
+// and their sums are added in wavefront order.
+constexpr int RED_WAVES = 16;
+
+__global__ __launch_bounds__(64 * RED_WAVES) void reduce_partials_kernel(int64_t n, int64_t nparts,
+                                                                         const double* __restrict__ part,
+                                                                         double* __restrict__ out) {
+  __shared__ double red[RED_WAVES][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
   double t = 0.0;
-  for (int64_t g = 0; g < nparts; ++g) t += part[g * n + c];
-  out[c] = t;
+  if (c < n) {
+#pragma unroll 8
+    for (int64_t g = wave; g < nparts; g += RED_WAVES) t += part[g * n + c];
+  }
+  red[wave][lane] = t;
+  __syncthreads();
+  if (wave == 0 && c < n) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < RED_WAVES; ++w) s += red[w][lane];
+    out[c] = s;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -452,22 +482,41 @@ __global__ __launch_bounds__(NP_THREADS) void residual_kernel(int64_t m, int64_t
   }
 }
 
-__global__ void residual_reduce_kernel(int64_t n, int64_t nparts, const double* __restrict__ part,
-                                       double* __restrict__ g, double* __restrict__ rss) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// g[c] = sum_q (s, comp) pairs of part in a fixed order (as reduce_partials_kernel,
+// pairs combined error-free); block n / 64 (one extra) also sums ||r||^2.
+__global__ __launch_bounds__(64 * RED_WAVES) void residual_reduce_kernel(int64_t n, int64_t nparts,
+                                                                         const double* __restrict__ part,
+                                                                         double* __restrict__ g,
+                                                                         double* __restrict__ rss) {
+  __shared__ double rs[RED_WAVES][64];
+  __shared__ double rc[RED_WAVES][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t ps = 2 * n + 2;
+  double s = 0.0, cc = 0.0;
   if (c < n) {
-    double s = 0.0, cc = 0.0;
-    for (int64_t q = 0; q < nparts; ++q) {
+    for (int64_t q = wave; q < nparts; q += RED_WAVES) {
       double t, e;
-      two_sum(s, part[q * (2 * n + 2) + c], t, e);
+      two_sum(s, part[q * ps + c], t, e);
       s = t;
-      cc = add_nc(cc, add_nc(part[q * (2 * n + 2) + n + c], e));
+      cc = add_nc(cc, add_nc(part[q * ps + n + c], e));
     }
-    g[c] = add_nc(s, cc);
   } else if (c == n && rss != nullptr) {
-    double t = 0.0;
-    for (int64_t q = 0; q < nparts; ++q) t += part[q * (2 * n + 2) + 2 * n];
-    rss[0] = t;
+    for (int64_t q = wave; q < nparts; q += RED_WAVES) s += part[q * ps + 2 * n];
+  }
+  rs[wave][lane] = s;
+  rc[wave][lane] = cc;
+  __syncthreads();
+  if (wave == 0) {
+    double S = 0.0, C = 0.0;
+    for (int w = 0; w < RED_WAVES; ++w) {
+      double t, e;
+      two_sum(S, rs[w][lane], t, e);
+      S = t;
+      C = add_nc(C, add_nc(rc[w][lane], e));
+    }
+    if (c < n) g[c] = add_nc(S, C);
+    else if (c == n && rss != nullptr) rss[0] = S;
   }
 }
 
@@ -555,14 +604,14 @@ inline int64_t np_grid(int64_t m) { return imax64(1, imin64(NP_MAX_GRID, pamd_cd
 // ---------------------------------------------------------------------------
 extern "C" int pamd_gauss_q_bf16(int64_t rows, int64_t m, uint64_t seed, uint64_t offset, uint16_t* Q, int64_t ldq,
                                  void* stream) {
-  PAMD_ARG_CHECK(rows >= 0, 1);
+  PAMD_ARG_CHECK(rows >= 0 && rows <= 65535, 1);
   PAMD_ARG_CHECK(m >= 0, 2);
   const int64_t kp = pamd_cdiv(m, OZ_CHUNK) * OZ_CHUNK;
   PAMD_ARG_CHECK(Q != nullptr || rows * kp == 0, 5);
   PAMD_ARG_CHECK(ldq >= kp && ldq % 8 == 0, 6);
   if (rows * kp == 0) return PAMD_OK;
-  gauss_q_kernel<<<(unsigned)pamd_cdiv(rows * kp, 256), 256, 0, (hipStream_t)stream>>>(rows, m, kp, seed, offset, Q,
-                                                                                       ldq);
+  dim3 grid((unsigned)pamd_cdiv(kp, 512), (unsigned)rows);
+  gauss_q_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(m, kp, seed, offset, Q, ldq);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }
@@ -663,7 +712,7 @@ extern "C" int pamd_normal_pass(int dtype, int64_t m, int64_t n, const void* B, 
 #undef PAMD_NP_T
 #undef PAMD_NP
   PAMD_LAUNCH_CHECK();
-  reduce_partials_kernel<<<(unsigned)pamd_cdiv(n, 256), 256, 0, st>>>(n, grid, part, q);
+  reduce_partials_kernel<<<(unsigned)pamd_cdiv(n, 64), 64 * RED_WAVES, 0, st>>>(n, grid, part, q);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }
@@ -699,7 +748,7 @@ extern "C" int pamd_residual_f64(int64_t m, int64_t n, const double* A, int64_t 
   else PAMD_RK(16);
 #undef PAMD_RK
   PAMD_LAUNCH_CHECK();
-  residual_reduce_kernel<<<(unsigned)pamd_cdiv(n + 1, 256), 256, 0, st>>>(n, grid, part, g, rss);
+  residual_reduce_kernel<<<(unsigned)pamd_cdiv(n + 1, 64), 64 * RED_WAVES, 0, st>>>(n, grid, part, g, rss);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

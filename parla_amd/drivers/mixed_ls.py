@@ -34,29 +34,30 @@ from parla_amd.comps.determiter.logging import SketchAndPrecondLog
 from parla_amd.drivers.least_squares import OverLstsqSolver, dim_checks
 
 
-def _dot(a, b):
-    return torch.dot(a, b)
-
-
 def cg_normal(op, rhs, max_iter, tol, check_every=8):
-    """z ~= (B^T B)^{-1} rhs by conjugate gradients; scalars stay on the device,
-    one host check of ||r|| every `check_every` iterations."""
-    z = torch.zeros_like(rhs)
-    r = rhs.clone()
-    p = r.clone()
-    rr = _dot(r, r)
-    stop = (tol * tol) * rr
+    """z ~= (B^T B)^{-1} rhs by conjugate gradients.  The recurrences (dots,
+    alpha, beta, the z / r / p updates) run in one device kernel per step
+    (pamd_bt_cg on a one-problem descriptor); the host reads ||r|| once every
+    `check_every` steps."""
+    n = rhs.numel()
+    dev = rhs.device
+    desc = torch.tensor([[1, n, 0, 0, 0, 0, 0, 0]], dtype=torch.int64, device=dev)
+    state = torch.tensor([0.0, 1.0, 0.0, 0.0], dtype=torch.float64, device=dev)
+    z, r, p, q = (torch.empty(n, dtype=torch.float64, device=dev) for _ in range(4))
+    L, st = ops.lib(), ops.stream(dev)
+
+    def step(mode):
+        ops.check(L.pamd_bt_cg(1, ops._p(desc), mode, ops._p(state), ops._p(rhs), ops._p(r), ops._p(p), ops._p(q),
+                               ops._p(z), st), "pamd_bt_cg")
+
+    step(0)
+    rr0 = state[0].clone()
     it = 0
     while it < max_iter:
-        q = op(p)
-        alpha = rr / _dot(p, q)
-        z.add_(alpha * p)
-        r.sub_(alpha * q)
-        rr_new = _dot(r, r)
-        p = r + (rr_new / rr) * p
-        rr = rr_new
+        op(p, q)
+        step(1)
         it += 1
-        if it % check_every == 0 and bool((rr <= stop).item()):
+        if it % check_every == 0 and bool((state[0] <= (tol * tol) * rr0).item()):
             break
     return z, it
 
