@@ -438,6 +438,64 @@ int pamd_bt_axpy(int64_t nprob, const pamd_ls_desc* descs, const double* state,
 int pamd_bt_copy(int64_t nprob, const pamd_ls_desc* descs, const double* a,
                  double* v, void* stream);
 
+/* ---- batched sketch and factorisation ------------------------------------
+ * The setup half of the batched solve: every problem's sketch Y_i = S_i A_i
+ * with its own SJLT (the column draws of pamd_sjlt_draw with seed_i), and the
+ * shifted CholeskyQR3 of Y_i, for the whole batch per launch.  Together they
+ * replace the per-problem sketch_op_gen + la.qr of drivers/least_squares.py:
+ * 302-311.  Work items are numbered problem after problem; pamd_sk_desc
+ * carries each problem's first item of every kind and its pool offsets
+ * (item counts: ceil(m_i / chunk_rows) chunks, d_i sketch rows, T(T+1)/2
+ * upper tiles with T = ceil(n_i / tile), ceil(d_i / strip_rows) strips,
+ * ceil(n_i / strip_rows) diagonal blocks; pamd_bt_layout returns the
+ * constants).  The caller builds the descriptors; the kernels trust them. */
+typedef struct pamd_sk_desc {
+  int64_t d;        /* sketch rows, vec_nnz <= d_i <= max_d (or d_i < vec_nnz: with replacement) */
+  int64_t y_off;    /* FP64 pool offset of Y_i (d_i x n_i, ld n_i) */
+  int64_t rp_off;   /* int64 row-pointer pool offset (d_i + 1 entries) */
+  int64_t cnt_off;  /* int32 chunk-histogram pool offset (chunks_i x d_i) */
+  int64_t chunk0, row0, tile0, strip0, dblk0;  /* first work item of each kind */
+  uint64_t seed;    /* SJLT seed of problem i */
+} pamd_sk_desc;
+
+int pamd_bt_layout(int* chunk_rows, int* tile, int* strip_rows, int* max_d, int* max_vec_nnz);
+
+/* CSR of every S_i (d_i x m_i, vec_nnz = k per column, entries +-1/sqrt(k)),
+ * each row in column order; problem i's entries start at k * mv_off_i of
+ * cols/vals and its row pointers (relative to that start) at rp_off_i.
+ * Replaces utils/sketching.py:34-80 for a batch. */
+int pamd_bt_sjlt_csr(int64_t nprob, const pamd_ls_desc* descs, const pamd_sk_desc* sk,
+                     int64_t nchunk, int k, int32_t* cnt, int64_t* rowptr, int32_t* cols,
+                     double* vals, void* stream);
+/* Y_i = S_i A_i (FP32 A pool, FP64 sums in column order): the sparse
+ * S @ A of drivers/least_squares.py:303, batched. */
+int pamd_bt_sjlt_left(int64_t nprob, const pamd_ls_desc* descs, const pamd_sk_desc* sk,
+                      int64_t nrows, int k, const int64_t* rowptr, const int32_t* cols,
+                      const double* vals, const float* A, double* Y, void* stream);
+/* G_i = Y_i^T Y_i into an n_i x n_i pool at r_off_i (upper tiles; the
+ * diagonal tiles whole), plus 11 (d n + n (n + 1)) u trace(G_i) on the
+ * diagonal when shifted = 1.  Factor G_i with pamd_potrf_batched_f64. */
+int pamd_bt_gram(int64_t nprob, const pamd_ls_desc* descs, const pamd_sk_desc* sk,
+                 int64_t ntiles, int shifted, const double* Y, double* G, void* stream);
+/* Y_i <- Y_i R_i^{-1} in place (R pool at r_off_i, upper, n_i <= nmax <= 512);
+ * dinv: 256 doubles per diagonal block (ndblk blocks). */
+int pamd_bt_trsm(int64_t nprob, const pamd_ls_desc* descs, const pamd_sk_desc* sk,
+                 int64_t nstrips, int64_t ndblk, int nmax, const double* R, double* dinv,
+                 double* Y, void* stream);
+/* C_i = A_i B_i for upper-triangular A_i, B_i (entries below the diagonal
+ * are not read); writes C's upper tiles, zeros below the diagonal.  Builds
+ * R = R_2 R_1 R_0 of CholeskyQR3. */
+int pamd_bt_trmm_upper(int64_t nprob, const pamd_ls_desc* descs, const pamd_sk_desc* sk,
+                       int64_t ntiles, const double* A, const double* B, double* C,
+                       void* stream);
+/* out[i] = ||A_i||_F^2 (FP32 pool, FP64 sums, fixed order); part: nchunk
+ * doubles.  The norm in the backward error, batched. */
+int pamd_bt_fro2(int64_t nprob, const pamd_ls_desc* descs, const pamd_sk_desc* sk,
+                 int64_t nchunk, const float* A, double* part, double* out, void* stream);
+/* out[i] = a_i . b_i over the n-vector pool. */
+int pamd_bt_dot(int64_t nprob, const pamd_ls_desc* descs, const double* a, const double* b,
+                double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,14 @@ SIGNATURES = {
     "pamd_bt_cg": (c_int, [c_i64, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_axpy": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_copy": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "pamd_bt_layout": (c_int, [ctypes.POINTER(c_int)] * 5),
+    "pamd_bt_sjlt_csr": (c_int, [c_i64, c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "pamd_bt_sjlt_left": (c_int, [c_i64, c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "pamd_bt_gram": (c_int, [c_i64, c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp]),
+    "pamd_bt_trsm": (c_int, [c_i64, c_vp, c_vp, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "pamd_bt_trmm_upper": (c_int, [c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "pamd_bt_fro2": (c_int, [c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "pamd_bt_dot": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_nrm2_large_f64": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp]),
     "pamd_trsm_right_upper_rec_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
     "pamd_mlp_nparams": (c_int, []),

@@ -11,7 +11,7 @@
 // each output column's CSR list through the tiles, so W's index arrays are
 // read once per 16 rows of A rather than once per row.
 #include "common.h"
-#include "rng.h"
+#include "sjlt_dev.h"
 
 namespace {
 
@@ -27,37 +27,20 @@ __global__ void gaussian_fill_kernel(int64_t rows, int64_t cols, double scale, u
 }
 
 // Column j: k distinct rows in [0, n_short) (rejection on repeats), signs.
+template <int KMAX>
 __global__ void sjlt_draw_kernel(int64_t n_short, int64_t n_long, int k, uint64_t seed, int32_t* __restrict__ rows,
                                  double* __restrict__ vals) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_long) return;
-  const bool with_repl = n_short < k;
   const double mag = 1.0 / sqrt((double)k);
-  int32_t picked[64];
-  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-  for (int l = 0; l < k; ++l) {
-    int32_t r = 0;
-    uint32_t sign = 0;
-    bool placed = false;
-    for (uint32_t attempt = 0; attempt < 64 && !placed; ++attempt) {
-      const U4 u = philox4x32_10(U4{(uint32_t)j, (uint32_t)(j >> 32), (uint32_t)l, 0x510E527Fu + attempt}, k0, k1);
-      r = (int32_t)(((uint64_t)u.x * (uint64_t)n_short) >> 32);
-      sign = u.y & 1u;
-      bool dup = false;
-      if (!with_repl)
-        for (int q = 0; q < l; ++q) dup |= (picked[q] == r);
-      placed = !dup;
-    }
-    if (!placed) {  // 64 straight repeats (practically unreachable): first unused row
-      for (r = 0; r < (int32_t)n_short; ++r) {
-        bool dup = false;
-        for (int q = 0; q < l; ++q) dup |= (picked[q] == r);
-        if (!dup) break;
-      }
-    }
-    picked[l] = r;
-    vals[j * k + l] = sign ? -mag : mag;
-    rows[j * k + l] = r;
+  int32_t picked[KMAX];
+  uint64_t signs;
+  sjlt_column<KMAX>(n_short, k, seed, j, picked, signs);
+#pragma unroll
+  for (int l = 0; l < KMAX; ++l) {
+    if (l >= k) continue;
+    vals[j * k + l] = ((signs >> l) & 1) ? -mag : mag;
+    rows[j * k + l] = picked[l];
   }
 }
 
@@ -81,41 +64,7 @@ __global__ __launch_bounds__(256) void csr_count_kernel(int64_t n_short, int64_t
 // rowptr = exclusive scan of row counts; cnt[b][r] <- start offset of block b in row r
 __global__ __launch_bounds__(1024) void csr_scan_kernel(int64_t n_short, int nblk, int32_t* __restrict__ cnt,
                                                         int64_t* __restrict__ rowptr) {
-  __shared__ int64_t wsum[16];
-  __shared__ int64_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int64_t base = 0; base < n_short; base += blockDim.x) {
-    const int64_t r = base + threadIdx.x;
-    int64_t c = 0;
-    if (r < n_short)
-      for (int b = 0; b < nblk; ++b) c += cnt[(int64_t)b * n_short + r];
-    // inclusive scan inside the wave
-    int64_t v = c;
-    for (int off = 1; off < 64; off <<= 1) {
-      const int64_t t = __shfl_up(v, off, 64);
-      if (lane >= off) v += t;
-    }
-    if (lane == 63) wsum[wave] = v;
-    __syncthreads();
-    int64_t wpre = 0;
-    for (int w = 0; w < wave; ++w) wpre += wsum[w];
-    const int64_t excl = carry + wpre + v - c;
-    if (r < n_short) {
-      rowptr[r] = excl;
-      int64_t run = excl;
-      for (int b = 0; b < nblk; ++b) {
-        const int32_t t = cnt[(int64_t)b * n_short + r];
-        cnt[(int64_t)b * n_short + r] = (int32_t)run;
-        run += t;
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) carry = excl + c;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) rowptr[n_short] = carry;
+  csr_scan_body(n_short, nblk, cnt, rowptr);
 }
 
 // Thread t owns rows r with r % 256 == t and places them in column order.
@@ -288,8 +237,11 @@ extern "C" int pamd_sjlt_draw(int64_t n_short, int64_t n_long, int k, uint64_t s
   PAMD_ARG_CHECK(rows != nullptr || n_long == 0, 5);
   PAMD_ARG_CHECK(vals != nullptr || n_long == 0, 6);
   if (n_long == 0) return PAMD_OK;
-  sjlt_draw_kernel<<<(unsigned)pamd_cdiv(n_long, 256), 256, 0, (hipStream_t)stream>>>(n_short, n_long, k, seed,
-                                                                                      rows, vals);
+  const unsigned grid = (unsigned)pamd_cdiv(n_long, 256);
+  if (k <= 16)
+    sjlt_draw_kernel<16><<<grid, 256, 0, (hipStream_t)stream>>>(n_short, n_long, k, seed, rows, vals);
+  else
+    sjlt_draw_kernel<64><<<grid, 256, 0, (hipStream_t)stream>>>(n_short, n_long, k, seed, rows, vals);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

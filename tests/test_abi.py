@@ -71,6 +71,20 @@ def test_argument_validation_without_gpu(built):
     assert lib.pamd_geqrf_workspace(10, 20, 20, ctypes.byref(b)) == -2  # n > m
     assert lib.pamd_geqrf_workspace(100, 20, 21, ctypes.byref(b)) == 0 and b.value > 0
     assert lib.pamd_sjlt_csr_workspace(20000, 100, ctypes.byref(b)) == -1  # n_short too large
+    # batched sketch: vec_nnz above the register-resident draw bound
+    assert lib.pamd_bt_sjlt_csr(1, ctypes.c_void_p(8), ctypes.c_void_p(8), 1, 17, *[ctypes.c_void_p(8)] * 4,
+                                None) == -5
+
+
+def test_batched_layout_constants(built):
+    """The constants the host-side batch plan is tested with are the library's."""
+    import ctypes
+    from parla_amd import _lib
+    import test_batched_plan as tbp
+    lib = _lib.load()
+    vals = [ctypes.c_int() for _ in range(5)]
+    assert lib.pamd_bt_layout(*[ctypes.byref(v) for v in vals]) == 0
+    assert tuple(v.value for v in vals) == tbp.LAYOUT
 
 
 def test_status_mapping():

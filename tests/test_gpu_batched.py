@@ -96,7 +96,10 @@ def test_batched_trsv_and_factors(bl):
         # R preconditions A: kappa(A R^-1) small
         s = np.linalg.svd(A.astype(np.float64) @ np.linalg.inv(R), compute_uv=False)
         assert s[0] / s[-1] < 10, (i, s[0] / s[-1])
-        np.testing.assert_allclose(R @ B.vec(B.t, i).cpu().numpy(), B.vec(v, i).cpu().numpy(), rtol=1e-8, atol=1e-9)
+        # the solve is backward stable: |R x - v| <= c u |R| |x| componentwise
+        x = B.vec(B.t, i).cpu().numpy()
+        res = np.abs(R @ x - B.vec(v, i).cpu().numpy())
+        assert (res <= 1e3 * 2.0 ** -53 * (np.abs(np.triu(R)) @ np.abs(x)) + 1e-300).all(), i
 
 
 def test_graph_replays_equal_eager(bl):
@@ -148,3 +151,83 @@ def test_batched_spo_solves_every_problem(bl):
         assert eta <= max(1e-10, 2 * eta_l), (i, eta, eta_l)
         s = np.linalg.svd(Ad, compute_uv=False)
         assert np.linalg.norm(x - xl) <= 1e-15 * (s[0] / s[-1]) * 1e3 * np.linalg.norm(xl), i
+
+
+def test_batched_sketch_matches_single_problem_path(bl):
+    """Y_i = S_i A_i from the batched CSR build and gather equals the
+    single-problem draw -> CSR -> left product with seed_i (same column draws,
+    same summation order)."""
+    _say("batched sketch vs single-problem sketch")
+    from parla_amd import ops
+    As, bs, _ = _problems(11, 21, mrange=(40, 9000), nrange=(1, 300))
+    As[0] = As[0][:6, :3].copy()  # d = min(m, 4 n) = 6 < vec_nnz: drawn with replacement
+    bs[0] = bs[0][:6].copy()
+    dev = torch.device("cuda")
+    B = bl.Batch(As, bs, dev)
+    solver = bl.BatchedSPO()
+    Y = solver.sketch(B, np.random.default_rng(22))
+    for i, A in enumerate(As):
+        m, n = A.shape
+        d = B.ds[i]
+        rows, vals = ops.sjlt_draw(d, m, 8, int(B.seeds[i]), dev)
+        rp, cl, cv = ops.sjlt_csr(d, m, 8, rows, vals)
+        ref = ops.sjlt_left(rp, cl, cv, d, torch.from_numpy(A).to(dev).double())
+        got = Y[B.sk_host[i, 1]:B.sk_host[i, 1] + d * n].view(d, n)
+        scale = torch.abs(ref).max().item()
+        assert torch.allclose(got, ref, rtol=0, atol=1e-15 * scale), (i, (got - ref).abs().max().item())
+
+
+def test_batched_factor_kernels_against_numpy(bl):
+    """Gram, triangular solve, triangular product, Frobenius norms and dots of
+    the batch against numpy per problem."""
+    _say("batched gram / trsm / trmm / fro2 / dot")
+    from parla_amd import ops
+    import ctypes
+    As, bs, _ = _problems(9, 23, mrange=(100, 3000), nrange=(1, 400))
+    dev = torch.device("cuda")
+    B = bl.Batch(As, bs, dev)
+    L, st = ops.lib(), ops.stream(dev)
+    D, S, tot, P = ops._p(B.desc), ops._p(B.sk), B.totals, B.P
+    rng = np.random.default_rng(24)
+    Y = torch.from_numpy(rng.standard_normal(tot["y"])).to(dev)
+    G = torch.full((tot["r"],), np.nan, dtype=torch.float64, device=dev)
+    ops.check(L.pamd_bt_gram(P, D, S, tot["tiles"], 0, ops._p(Y), ops._p(G), st), "gram")
+    # well-conditioned upper triangles for the solve and the product
+    Ra = torch.from_numpy(rng.standard_normal(tot["r"])).to(dev)
+    Rb = torch.from_numpy(rng.standard_normal(tot["r"])).to(dev)
+    for i, n in enumerate(B.ns):
+        for Rp in (Ra, Rb):
+            v = Rp[B.r_off[i]:B.r_off[i + 1]].view(n, n)
+            v.diagonal().copy_(n + torch.rand(n, dtype=torch.float64, device=dev))
+    C = torch.zeros(tot["r"], dtype=torch.float64, device=dev)
+    ops.check(L.pamd_bt_trmm_upper(P, D, S, tot["tiles"], ops._p(Ra), ops._p(Rb), ops._p(C), st), "trmm")
+    X = Y.clone()
+    dinv = torch.empty(256 * tot["dblk"], dtype=torch.float64, device=dev)
+    ops.check(L.pamd_bt_trsm(P, D, S, tot["strips"], tot["dblk"], max(B.ns), ops._p(Ra), ops._p(dinv),
+                             ops._p(X), st), "trsm")
+    dots = torch.empty(P, dtype=torch.float64, device=dev)
+    ops.check(L.pamd_bt_dot(P, D, ops._p(B.x.copy_(torch.from_numpy(rng.standard_normal(B.x.numel())))),
+                            ops._p(B.t.copy_(torch.from_numpy(rng.standard_normal(B.t.numel())))), ops._p(dots),
+                            st), "dot")
+    import scipy.linalg as sla
+    for i, A in enumerate(As):
+        m, n = A.shape
+        d = B.ds[i]
+        yo = B.sk_host[i, 1]
+        Yi = Y[yo:yo + d * n].view(d, n).cpu().numpy()
+        Gi = G[B.r_off[i]:B.r_off[i + 1]].view(n, n).cpu().numpy()
+        Gref = Yi.T @ Yi
+        iu = np.triu_indices(n)
+        np.testing.assert_allclose(Gi[iu], Gref[iu], rtol=0, atol=1e-13 * np.abs(Gref).max())
+        Ai = np.triu(Ra[B.r_off[i]:B.r_off[i + 1]].view(n, n).cpu().numpy())
+        Bi = np.triu(Rb[B.r_off[i]:B.r_off[i + 1]].view(n, n).cpu().numpy())
+        Ci = C[B.r_off[i]:B.r_off[i + 1]].view(n, n).cpu().numpy()
+        np.testing.assert_allclose(Ci, Ai @ Bi, rtol=0, atol=1e-13 * np.abs(Ai @ Bi).max())
+        Xi = X[yo:yo + d * n].view(d, n).cpu().numpy()
+        Xref = sla.solve_triangular(Ai, Yi.T, trans="T").T
+        np.testing.assert_allclose(Xi, Xref, rtol=0, atol=1e-13 * np.abs(Xref).max())
+        Ad = A.astype(np.float64)
+        assert abs(B.anorm[i].item() - np.linalg.norm(Ad)) <= 1e-14 * np.linalg.norm(Ad)
+        xv = B.vec(B.x, i).cpu().numpy()
+        tv = B.vec(B.t, i).cpu().numpy()
+        assert abs(dots[i].item() - xv @ tv) <= 1e-13 * (np.abs(xv) @ np.abs(tv))
