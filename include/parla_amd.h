@@ -404,14 +404,17 @@ typedef struct pamd_ls_block {
  *   normal:   partial of A_i^T (A_i p_i) per block;
  *   residual: partial of the compensated g_i = A_i^T (b_i - A_i x_i) and
  *             ||r_i||^2 per block.  Replace LSQR's matvecs
- *             (comps/determiter/lsqr.py:345-380) for a whole batch. */
+ *             (comps/determiter/lsqr.py:345-380) for a whole batch.
+ * state (the CG state of pamd_bt_cg, or NULL): blocks of problems whose
+ * active flag is 0 are skipped and keep their previous partials. */
 int pamd_bt_normal(int npl, const pamd_ls_desc* descs, const pamd_ls_block* blocks,
                    int64_t nblk, int waves_per_wg, int blocks_per_wave,
-                   const float* A, const double* p, double* part, void* stream);
+                   const float* A, const double* p, const double* state, double* part,
+                   void* stream);
 int pamd_bt_residual(int npl, const pamd_ls_desc* descs, const pamd_ls_block* blocks,
                      int64_t nblk, int waves_per_wg, int blocks_per_wave,
                      const float* A, const double* b, const double* x,
-                     double* part, void* stream);
+                     const double* state, double* part, void* stream);
 
 /* Per-problem sum of the block partials in block order into out (n-vector
  * pool); compensated = 1 also folds the compensation terms and writes

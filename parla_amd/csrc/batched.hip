@@ -50,13 +50,14 @@ template <int NPL>
 __global__ __launch_bounds__(256) void bt_normal_kernel(const Desc* __restrict__ D, const Blk* __restrict__ blks,
                                                         int64_t nblk, int blocks_per_wave,
                                                         const float* __restrict__ A, const double* __restrict__ pv,
-                                                        double* __restrict__ part) {
+                                                        const double* __restrict__ state, double* __restrict__ part) {
   const int lane = threadIdx.x & 63;
   const int64_t wid = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   for (int k = 0; k < blocks_per_wave; ++k) {
     const int64_t bi = wid * blocks_per_wave + k;
     if (bi >= nblk) return;
     const Blk bk = blks[bi];
+    if (state != nullptr && state[4 * bk.prob + 1] == 0.0) continue;  // frozen problem: partial stays
     const Desc d = D[bk.prob];
     const int64_t n = d.n;
     const float* Ai = A + d.a_off;
@@ -130,13 +131,15 @@ template <int NPL>
 __global__ __launch_bounds__(256) void bt_residual_kernel(const Desc* __restrict__ D, const Blk* __restrict__ blks,
                                                           int64_t nblk, int blocks_per_wave,
                                                           const float* __restrict__ A, const double* __restrict__ b,
-                                                          const double* __restrict__ xv, double* __restrict__ part) {
+                                                          const double* __restrict__ xv,
+                                                          const double* __restrict__ state, double* __restrict__ part) {
   const int lane = threadIdx.x & 63;
   const int64_t wid = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   for (int k = 0; k < blocks_per_wave; ++k) {
     const int64_t bi = wid * blocks_per_wave + k;
     if (bi >= nblk) return;
     const Blk bk = blks[bi];
+    if (state != nullptr && state[4 * bk.prob + 1] == 0.0) continue;  // frozen problem: partial stays
     const Desc d = D[bk.prob];
     const int64_t n = d.n;
     const float* Ai = A + d.a_off;
@@ -733,8 +736,8 @@ static int bt_shape(int npl, int64_t nblk, int waves_per_wg, int blocks_per_wave
 }
 
 extern "C" int pamd_bt_normal(int npl, const pamd_ls_desc* descs, const pamd_ls_block* blocks, int64_t nblk,
-                              int waves_per_wg, int blocks_per_wave, const float* A, const double* p, double* part,
-                              void* stream) {
+                              int waves_per_wg, int blocks_per_wave, const float* A, const double* p,
+                              const double* state, double* part, void* stream) {
   unsigned grid, threads;
   const int rc = bt_shape(npl, nblk, waves_per_wg, blocks_per_wave, &grid, &threads);
   PAMD_ARG_CHECK(rc != -1, 1);
@@ -745,14 +748,15 @@ extern "C" int pamd_bt_normal(int npl, const pamd_ls_desc* descs, const pamd_ls_
   PAMD_ARG_CHECK(rc != -3, 6);
   if (nblk == 0) return PAMD_OK;
   hipStream_t st = (hipStream_t)stream;
-  PAMD_BT_NPL(bt_normal_kernel, npl, (const Desc*)descs, (const Blk*)blocks, nblk, blocks_per_wave, A, p, part)
+  PAMD_BT_NPL(bt_normal_kernel, npl, (const Desc*)descs, (const Blk*)blocks, nblk, blocks_per_wave, A, p, state,
+              part)
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }
 
 extern "C" int pamd_bt_residual(int npl, const pamd_ls_desc* descs, const pamd_ls_block* blocks, int64_t nblk,
                                 int waves_per_wg, int blocks_per_wave, const float* A, const double* b,
-                                const double* x, double* part, void* stream) {
+                                const double* x, const double* state, double* part, void* stream) {
   unsigned grid, threads;
   const int rc = bt_shape(npl, nblk, waves_per_wg, blocks_per_wave, &grid, &threads);
   PAMD_ARG_CHECK(rc != -1, 1);
@@ -763,7 +767,8 @@ extern "C" int pamd_bt_residual(int npl, const pamd_ls_desc* descs, const pamd_l
   PAMD_ARG_CHECK(rc != -3, 6);
   if (nblk == 0) return PAMD_OK;
   hipStream_t st = (hipStream_t)stream;
-  PAMD_BT_NPL(bt_residual_kernel, npl, (const Desc*)descs, (const Blk*)blocks, nblk, blocks_per_wave, A, b, x, part)
+  PAMD_BT_NPL(bt_residual_kernel, npl, (const Desc*)descs, (const Blk*)blocks, nblk, blocks_per_wave, A, b, x,
+              state, part)
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

@@ -279,11 +279,13 @@ class BatchedSPO:
             if kind == "normal":
                 def launch(w, k, c=c, blks=blks, nb=nb):
                     ops.check(L.pamd_bt_normal(c, ops._p(B.desc), ops._p(blks), nb, w, k, ops._p(B.A),
-                                               ops._p(B.t), ops._p(B.part), st), "pamd_bt_normal")
+                                               ops._p(B.t), ops._p(B.state), ops._p(B.part), st),
+                              "pamd_bt_normal")
             else:
                 def launch(w, k, c=c, blks=blks, nb=nb):
                     ops.check(L.pamd_bt_residual(c, ops._p(B.desc), ops._p(blks), nb, w, k, ops._p(B.A),
-                                                 ops._p(B.b), ops._p(B.x), ops._p(B.part), st), "pamd_bt_residual")
+                                                 ops._p(B.b), ops._p(B.x), ops._p(B.state), ops._p(B.part), st),
+                              "pamd_bt_residual")
             w, k = self.tuner.choose((kind, c, nb), launch)
             launch(w, k)
 
@@ -347,8 +349,8 @@ class BatchedSPO:
         if use_graph:
             # tune the normal pass outside the capture; nothing in a step
             # allocates or synchronises.  The first call also runs one step
-            # with every problem frozen (x stays, g and rss are recomputed
-            # bit for bit) so that each kernel has been launched once.
+            # with every problem frozen (every pass skips them, x stays) so
+            # that each kernel has been launched once before the capture.
             self._copy(B, B.p, B.t)
             self.normal(B, B.q)
             if not self._warm:
@@ -356,6 +358,9 @@ class BatchedSPO:
                 self.refine_step(B)
                 B.state[1::4] = 1.0
                 self._warm = True
+            # the partials of frozen problems must be their residual's: a
+            # problem skipped by later passes keeps g_i and ||r_i|| this way
+            self.residual(B)
             graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(graph):
                 self.refine_step(B)

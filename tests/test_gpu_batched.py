@@ -231,3 +231,27 @@ def test_batched_factor_kernels_against_numpy(bl):
         xv = B.vec(B.x, i).cpu().numpy()
         tv = B.vec(B.t, i).cpu().numpy()
         assert abs(dots[i].item() - xv @ tv) <= 1e-13 * (np.abs(xv) @ np.abs(tv))
+
+
+def test_frozen_problems_are_untouched(bl):
+    """A refinement step with some problems frozen leaves their x, g and
+    ||r||^2 bit for bit as they were (their row blocks are skipped), and
+    moves the active ones."""
+    _say("frozen problems")
+    As, bs, _ = _problems(8, 31)
+    dev = torch.device("cuda")
+    solver = bl.BatchedSPO(cg_iters=5)
+    B = bl.Batch(As, bs, dev)
+    solver.factor(B, np.random.default_rng(32))
+    solver.residual(B)
+    solver.refine_step(B)  # all active: x moves off zero
+    x0, g0, rss0 = B.x.clone(), B.g.clone(), B.rss.clone()
+    active = np.arange(B.P) % 2 == 0
+    B.state[1::4] = torch.from_numpy(active.astype(np.float64)).to(dev)
+    solver.refine_step(B)
+    for i in range(B.P):
+        same = torch.equal(B.vec(B.x, i), B.vec(x0, i)) and torch.equal(B.vec(B.g, i), B.vec(g0, i))
+        if active[i]:
+            assert not torch.equal(B.vec(B.x, i), B.vec(x0, i)), i
+        else:
+            assert same and B.rss[i].item() == rss0[i].item(), i
