@@ -499,6 +499,32 @@ int pamd_bt_fro2(int64_t nprob, const pamd_ls_desc* descs, const pamd_sk_desc* s
 int pamd_bt_dot(int64_t nprob, const pamd_ls_desc* descs, const double* a, const double* b,
                 double* out, void* stream);
 
+/* ---- split-precision GEMM (BF16 pieces on BF16 MFMA) ----------------------
+ * An FP64 operand is held as npieces (2 or 3) BF16 pieces x = x_0 + x_1 (+ x_2),
+ * x_p the BF16 rounding of what x_0 .. x_{p-1} leave; C = L R^T sums the
+ * piece products L_p R_q^T with p + q < npieces in FP32 on
+ * v_mfma_f32_16x16x32_bf16 (split-K partials summed in FP64).  16 (24)
+ * significant bits per element for 2 (3) pieces.  It runs the products of
+ * the power iteration and of B = Q^T A (comps/sketchers/aware.py:175,179;
+ * comps/rangefinders.py:186; comps/qb.py:351) in BF16 arithmetic.
+ * Piece arrays: npieces x rows x ldk uint16 (BF16 bits), piece stride
+ * pstride, K-contiguous; columns K..kp-1 zero. */
+
+/* kp (column padding of every piece array, a multiple of 64), npad (rows of
+ * R's piece array, zero beyond N) and the workspace of pamd_split_gemm_f64. */
+int pamd_split_layout(int npieces, int64_t M, int64_t N, int64_t K, int64_t* kp,
+                      int64_t* npad, int64_t* workspace_bytes);
+/* Pieces of X (rows x cols, row-major FP64) into rows_out x kp arrays, or of
+ * X^T (cols x rows) with transpose = 1; zero outside X. */
+int pamd_split_bf16(int transpose, int npieces, int64_t rows, int64_t cols, const double* X,
+                    int64_t ldx, int64_t rows_out, int64_t kp, uint16_t* pieces, int64_t ldk,
+                    int64_t pstride, void* stream);
+/* C[i * lci + j * lcj] = sum_{p+q<npieces} sum_k L_p[i][k] R_q[j][k] for
+ * i < M, j < N (lcj = 1: row-major C; lci = 1: C stored transposed). */
+int pamd_split_gemm_f64(int npieces, int64_t M, int64_t N, int64_t K, const uint16_t* L,
+                        int64_t ldl, int64_t lps, const uint16_t* R, int64_t ldr, int64_t rps,
+                        double* C, int64_t lci, int64_t lcj, void* workspace, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,8 @@ Parity status of each function (how it is pinned, see tests/test_oracle_*.py):
 * mlp_forward, mlp_loss_grad -- central finite differences of the loss;
   bf16_round -- exhaustive comparison with torch's bfloat16 rounding;
   leverage_scores -- sum = n, invariance under right scaling, closed forms.
+* bf16_pieces, split_product -- the reconstruction bound of the pieces, and
+  exact rational evaluation of the piece products on small cases.
 """
 import time
 import warnings
@@ -50,7 +52,7 @@ __all__ = [
     "loglog_fit", "backward_error", "EPS", "quantize_q", "ozaki_split", "ozaki_product",
     "residual_extended", "residual_exact", "backward_error_extended", "spo_mixed", "bf16_round",
     "mlp_unpack", "mlp_effective", "mlp_forward", "mlp_loss_grad", "leverage_scores", "coherent_matrix",
-    "MLP_NPARAM",
+    "MLP_NPARAM", "bf16_pieces", "bf16_values", "split_product",
 ]
 
 
@@ -747,6 +749,40 @@ def bf16_round(x):
     u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
     u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
     return u.astype(np.uint32).view(np.float32)
+
+
+def bf16_pieces(X, npieces):
+    """The BF16 pieces of FP64 values, as uint16 bits, shape (npieces,) + X.shape:
+    piece p is the BF16 rounding (through FP32, ties to even) of X minus the
+    earlier pieces, the difference taken exactly in FP64.  No reference
+    counterpart (the split-precision QB products of this build).  Pinned by
+    the reconstruction bound |X - sum| <= 2^-(8 npieces) |X| and by
+    bf16_round (tests/test_oracle.py)."""
+    r = np.array(X, dtype=np.float64, copy=True)
+    out = []
+    for _ in range(npieces):
+        h = bf16_round(r.astype(np.float32))
+        out.append((h.view(np.uint32) >> 16).astype(np.uint16))
+        r = r - h.astype(np.float64)
+    return np.stack(out)
+
+
+def bf16_values(bits):
+    """FP64 values of BF16 bit patterns."""
+    return (np.asarray(bits, dtype=np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def split_product(Lbits, Rbits):
+    """sum over p + q < P of L_p R_q^T in FP64 from piece bit arrays
+    (P x M x K and P x N x K): the product the split GEMM forms (it
+    accumulates in FP32, so the two differ by rounding).  Pinned against
+    exact rational arithmetic on small cases (tests/test_oracle.py)."""
+    P = Lbits.shape[0]
+    C = np.zeros((Lbits.shape[1], Rbits.shape[1]))
+    for p in range(P):
+        for q in range(P - p):
+            C += bf16_values(Lbits[p]) @ bf16_values(Rbits[q]).T
+    return C
 
 
 def mlp_unpack(theta):

@@ -24,7 +24,7 @@ def __getattr__(name):
         'RF1': 'parla_amd.comps.rangefinders', 'RangeFinder': 'parla_amd.comps.rangefinders',
         'QB1': 'parla_amd.comps.qb', 'QBDecomposer': 'parla_amd.comps.qb',
         'SPO': 'parla_amd.drivers.least_squares', 'OverLstsqSolver': 'parla_amd.drivers.least_squares',
-        'SPOMixed': 'parla_amd.drivers.mixed_ls',
+        'SPOMixed': 'parla_amd.drivers.mixed_ls', 'SplitOperator': 'parla_amd.utils.split_operator',
         'SketchAndPrecondLog': 'parla_amd.comps.determiter.logging',
         'SVD1': 'parla_amd.drivers.svd', 'SVDecomposer': 'parla_amd.drivers.svd',
         'EVD1': 'parla_amd.drivers.evd', 'EVDecomposer': 'parla_amd.drivers.evd',

@@ -2,6 +2,7 @@
 import numpy as np
 
 from parla_amd import ops
+import parla_amd.utils.sketching as usk
 from parla_amd.comps.rangefinders import RangeFinder, RF1  # noqa: F401
 from parla_amd.comps.sketchers.aware import RS1  # noqa: F401
 
@@ -32,5 +33,5 @@ class QB1(QBDecomposer):
         rng = np.random.default_rng(rng)
         At = ops.as_device(A)
         Q = self.rangefinder(At, k, tol, rng)
-        B = ops.gemm(Q.t(), At)
+        B = usk.apply(Q.t(), At)
         return ops.like_input(Q, A), ops.like_input(B, A)

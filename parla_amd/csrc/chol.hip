@@ -21,6 +21,7 @@ namespace {
 constexpr int CB = 32;
 constexpr int POTRF_MAX_N = 512;  // block row of R staged in LDS: 32 x 512 doubles
 constexpr int POTRF_THREADS = 512;
+constexpr int SYRK_G = 4;  // trailing-update tiles per wavefront whose loads are issued together
 
 
 // Blocked right-looking upper Cholesky in one 512-thread workgroup, 32-wide
@@ -135,35 +136,49 @@ __device__ __forceinline__ void potrf_block(int64_t n, double* __restrict__ A, i
       A[(jb + i) * lda + c0 + c] = Rs[i][c];
     }
     // 3. trailing SYRK on upper-triangle tiles (ti <= tc); tile index t
-    //    enumerates row ti's tiles tc = ti .. T-1 in order
+    //    enumerates row ti's tiles tc = ti .. T-1 in order.  A wavefront
+    //    takes its tiles SYRK_G at a time and issues the group's global loads
+    //    together, so it waits for memory once per group, not once per tile.
     const int T = ntp >> 4;
     const int ntiles = T * (T + 1) / 2;
-    for (int t = wave; t < ntiles; t += nwaves) {
-      int ti = 0, rem = t;
-      while (rem >= T - ti) {
-        rem -= T - ti;
-        ++ti;
+    for (int t0 = wave; t0 < ntiles; t0 += SYRK_G * nwaves) {
+      int tis[SYRK_G], tcs[SYRK_G];
+      double cv[SYRK_G][4];
+#pragma unroll
+      for (int g = 0; g < SYRK_G; ++g) {
+        const int t = t0 + g * nwaves;
+        int ti = 0, rem = (t < ntiles) ? t : 0;
+        while (rem >= T - ti) {
+          rem -= T - ti;
+          ++ti;
+        }
+        tis[g] = (t < ntiles) ? ti : -1;
+        tcs[g] = ti + rem;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = ti * 16 + lk + 4 * r, cc = tcs[g] * 16 + lr;
+          cv[g][r] = (t < ntiles && i < nt && cc < nt && cc >= i) ? A[(c0 + i) * lda + c0 + cc] : 0.0;
+        }
       }
-      const int tc = ti + rem;
-      double cv[4], av[CB / 4], bv[CB / 4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = ti * 16 + lk + 4 * r, cc = tc * 16 + lr;
-        cv[r] = (i < nt && cc < nt && cc >= i) ? A[(c0 + i) * lda + c0 + cc] : 0.0;
-      }
+      for (int g = 0; g < SYRK_G; ++g) {
+        if (tis[g] < 0) continue;  // wave-uniform
+        const int ti = tis[g], tc = tcs[g];
+        double av[CB / 4], bv[CB / 4];
 #pragma unroll
-      for (int q = 0; q < CB / 4; ++q) {
-        av[q] = Rs[4 * q + lk][ti * 16 + lr];
-        bv[q] = Rs[4 * q + lk][tc * 16 + lr];
-      }
-      __builtin_amdgcn_sched_barrier(0);
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < CB / 4; ++q) {
+          av[q] = Rs[4 * q + lk][ti * 16 + lr];
+          bv[q] = Rs[4 * q + lk][tc * 16 + lr];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int q = 0; q < CB / 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
+        for (int q = 0; q < CB / 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = ti * 16 + lk + 4 * r, cc = tc * 16 + lr;
-        if (i < nt && cc < nt && cc >= i) A[(c0 + i) * lda + c0 + cc] = cv[r] - acc[r];
+        for (int r = 0; r < 4; ++r) {
+          const int i = ti * 16 + lk + 4 * r, cc = tc * 16 + lr;
+          if (i < nt && cc < nt && cc >= i) A[(c0 + i) * lda + c0 + cc] = cv[g][r] - acc[r];
+        }
       }
     }
     __syncthreads();

@@ -66,7 +66,11 @@ def _p(t):
 
 
 def as_device(X, dev=None):
-    """float64 CUDA tensor view of X (numpy array, CPU or CUDA tensor)."""
+    """float64 CUDA tensor view of X (numpy array, CPU or CUDA tensor); an
+    operator that holds its matrix on the device in another form (the split
+    operator) passes through unchanged."""
+    if getattr(X, "device_operator", False):
+        return X
     if isinstance(X, torch.Tensor):
         if X.device.type == "cuda":
             return X if X.dtype == F64 else X.to(F64)
@@ -77,8 +81,9 @@ def as_device(X, dev=None):
 
 
 def like_input(t, ref):
-    """Return t in the kind of `ref`: CUDA tensor stays, anything else -> numpy."""
-    if isinstance(ref, torch.Tensor) and ref.device.type == "cuda":
+    """Return t in the kind of `ref`: CUDA tensor (or device operator) stays,
+    anything else -> numpy."""
+    if getattr(ref, "device_operator", False) or (isinstance(ref, torch.Tensor) and ref.device.type == "cuda"):
         return t
     if isinstance(t, torch.Tensor):
         return t.detach().cpu().numpy()
@@ -504,6 +509,66 @@ def trsm_rec_(X, R):
     check(lib().pamd_trsm_right_upper_rec_f64(m, n, _p(R), max(R.stride(0), n, 1), _p(X), max(X.stride(0), n, 1),
                                               _p(ws), ws_bytes, stream(X.device)), "pamd_trsm_right_upper_rec_f64")
     return X
+
+
+# ---------------------------------------------------------------------------
+# Split-precision GEMM: FP64 operands as BF16 pieces on BF16 MFMA
+# ---------------------------------------------------------------------------
+def split_layout(npieces, M, N, K):
+    """(kp, npad, workspace bytes) of the split GEMM C (M x N) = L R^T, K
+    reduced: pieces need kp >= K columns and R needs npad >= N rows."""
+    kp, npad, ws = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    check(lib().pamd_split_layout(npieces, M, N, K, ctypes.byref(kp), ctypes.byref(npad), ctypes.byref(ws)),
+          "pamd_split_layout")
+    return kp.value, npad.value, ws.value
+
+
+class SplitPieces:
+    """BF16 pieces (npieces x rows_out x kp, uint16 bits) of a K-contiguous
+    operand: `rows` logical rows, K logical columns."""
+
+    def __init__(self, pieces, rows, K):
+        self.pieces = pieces
+        self.rows = rows
+        self.K = K
+
+    @property
+    def npieces(self):
+        return self.pieces.shape[0]
+
+
+def split_bf16(X, npieces, transpose=False, rows_out=None, kp=None):
+    """Pieces of X (row-major FP64, rows x cols) or, with transpose, of X^T."""
+    Xc = X if X.stride(-1) == 1 else X.contiguous()
+    rows, cols = Xc.shape
+    out_rows, K = (cols, rows) if transpose else (rows, cols)
+    kp = kp or _round_up(max(K, 1), 64)
+    rows_out = rows_out or out_rows
+    P = torch.empty((npieces, rows_out, kp), dtype=torch.uint16, device=X.device)
+    check(lib().pamd_split_bf16(int(transpose), npieces, rows, cols, _p(Xc), max(Xc.stride(0), cols, 1), rows_out,
+                                kp, _p(P), kp, rows_out * kp, stream(X.device)), "pamd_split_bf16")
+    return SplitPieces(P, out_rows, K)
+
+
+def split_gemm(L, R, out=None, transpose_out=False):
+    """C = sum_{p+q<P} L_p R_q^T (M x N, FP64) from SplitPieces L (M x K) and
+    R (N x K, its piece array padded to the layout's npad rows); with
+    transpose_out the result is written as C^T (N x M)."""
+    npieces = L.npieces
+    M, N, K = L.rows, R.rows, L.K
+    kp, npad, wsb = split_layout(npieces, M, N, K)
+    if L.pieces.shape[2] < kp or R.pieces.shape[2] < kp or R.pieces.shape[1] < npad or R.npieces != npieces:
+        raise ValueError("split_gemm: piece arrays do not match the layout")
+    dev = L.pieces.device
+    if out is None:
+        out = torch.empty((N, M) if transpose_out else (M, N), dtype=F64, device=dev)
+    lci, lcj = (1, out.stride(0)) if transpose_out else (out.stride(0), 1)
+    ws = workspace(wsb, dev)
+    Lp, Rp = L.pieces, R.pieces
+    check(lib().pamd_split_gemm_f64(npieces, M, N, K, _p(Lp), Lp.shape[2], Lp.shape[1] * Lp.shape[2], _p(Rp),
+                                    Rp.shape[2], Rp.shape[1] * Rp.shape[2], _p(out), lci, lcj, _p(ws),
+                                    stream(dev)), "pamd_split_gemm_f64")
+    return out
 
 
 def sjlt_draw(n_short, n_long, k, seed, dev):

@@ -301,6 +301,10 @@ def apply(X, Y):
     library's kernels rather than torch's.
     """
     ops_types = (DenseSketch, SJLTSketch, SamplingSketch)
+    if getattr(X, "device_operator", False):  # split operator: sketches see its FP64 matrix
+        return Y.__rmatmul__(X.dense()) if isinstance(Y, ops_types) else X @ Y
+    if getattr(Y, "device_operator", False):
+        return X.__matmul__(Y.dense()) if isinstance(X, ops_types) else Y.__rmatmul__(X)
     if isinstance(Y, ops_types):
         return Y.__rmatmul__(X)
     if isinstance(X, ops_types):

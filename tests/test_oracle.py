@@ -460,3 +460,43 @@ def test_leverage_scores_invariants():
     # closed form: orthonormal columns -> leverage = squared row norms
     Q = la.qr(rng.standard_normal((200, 7)), mode="economic")[0]
     np.testing.assert_allclose(orc.leverage_scores(Q), np.sum(Q ** 2, axis=1), rtol=1e-12)
+
+
+def test_bf16_pieces_reconstruct_and_round_to_nearest():
+    """Each piece is the nearest BF16 of what the earlier pieces leave, and
+    P pieces carry 8 P bits: |x - sum| <= 2^-8P |x| (BF16 rounds to within
+    2^-8 relative)."""
+    import torch
+    rng = np.random.default_rng(41)
+    x = rng.standard_normal(20000) * np.exp(rng.uniform(-30, 30, 20000))
+    for P in (1, 2, 3):
+        bits = orc.bf16_pieces(x, P)
+        vals = orc.bf16_values(bits)
+        r = x.copy()
+        for p in range(P):
+            ref = torch.from_numpy(r.astype(np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+            assert np.array_equal(vals[p], ref)
+            r = r - vals[p]
+        assert np.all(np.abs(x - vals.sum(0)) <= 2.0 ** -(8 * P) * np.abs(x))
+        assert np.array_equal(np.abs(r), np.abs(x - vals.sum(0)))
+
+
+def test_split_product_matches_exact_rational_sum():
+    """split_product equals the exactly evaluated sum of the piece products
+    with p + q < P (and only those)."""
+    from fractions import Fraction
+    rng = np.random.default_rng(42)
+    L = rng.standard_normal((3, 7))
+    R = rng.standard_normal((4, 7))
+    for P in (2, 3):
+        Lb, Rb = orc.bf16_pieces(L, P), orc.bf16_pieces(R, P)
+        C = orc.split_product(Lb, Rb)
+        Lv, Rv = orc.bf16_values(Lb), orc.bf16_values(Rb)
+        for i in range(3):
+            for j in range(4):
+                ex = sum(Fraction(float(Lv[p, i, k])) * Fraction(float(Rv[q, j, k]))
+                         for p in range(P) for q in range(P) if p + q < P for k in range(7))
+                assert abs(Fraction(float(C[i, j])) - ex) <= Fraction(2.0 ** -50) * abs(ex) + Fraction(2.0 ** -300)
+        full = L @ R.T
+        # dropped products and piece truncation are below 2^-(8P-1) |L| |R|^T elementwise
+        assert np.all(np.abs(C - full) <= 2.0 ** -(8 * P - 2) * (np.abs(L) @ np.abs(R).T))
