@@ -507,23 +507,26 @@ int pamd_bt_dot(int64_t nprob, const pamd_ls_desc* descs, const double* a, const
  * significant bits per element for 2 (3) pieces.  It runs the products of
  * the power iteration and of B = Q^T A (comps/sketchers/aware.py:175,179;
  * comps/rangefinders.py:186; comps/qb.py:351) in BF16 arithmetic.
- * Piece arrays: npieces x rows x ldk uint16 (BF16 bits), piece stride
- * pstride, K-contiguous; columns K..kp-1 zero. */
+ * Piece arrays: npieces arrays (stride pstride elements) of rows x kp BF16
+ * (uint16 bits) in 1 KiB blocks of 16 rows x 32 columns, element (i, k) at
+ *   ((i/16)(kp/32) + k/32) 512 + (i%16) 32 + 8 (c ^ ((i%16)/4)) + k%8,
+ * c = (k%32)/8; zero beyond the matrix.  kp is a multiple of 64. */
 
-/* kp (column padding of every piece array, a multiple of 64), npad (rows of
- * R's piece array, zero beyond N) and the workspace of pamd_split_gemm_f64. */
+/* kp, the rows of L's (mpad) and R's (npad) piece arrays, and the workspace
+ * of pamd_split_gemm_f64, for C (M x N) with K reduced. */
 int pamd_split_layout(int npieces, int64_t M, int64_t N, int64_t K, int64_t* kp,
-                      int64_t* npad, int64_t* workspace_bytes);
-/* Pieces of X (rows x cols, row-major FP64) into rows_out x kp arrays, or of
- * X^T (cols x rows) with transpose = 1; zero outside X. */
+                      int64_t* mpad, int64_t* npad, int64_t* workspace_bytes);
+/* Pieces of X (rows x cols, row-major FP64) into rows_out x kp arrays
+ * (rows_out a multiple of 16), or of X^T (cols x rows) with transpose = 1. */
 int pamd_split_bf16(int transpose, int npieces, int64_t rows, int64_t cols, const double* X,
-                    int64_t ldx, int64_t rows_out, int64_t kp, uint16_t* pieces, int64_t ldk,
+                    int64_t ldx, int64_t rows_out, int64_t kp, uint16_t* pieces,
                     int64_t pstride, void* stream);
 /* C[i * lci + j * lcj] = sum_{p+q<npieces} sum_k L_p[i][k] R_q[j][k] for
- * i < M, j < N (lcj = 1: row-major C; lci = 1: C stored transposed). */
+ * i < M, j < N (lcj = 1: row-major C; lci = 1: C stored transposed); L has
+ * mpad rows, R npad rows, both kp columns (pamd_split_layout). */
 int pamd_split_gemm_f64(int npieces, int64_t M, int64_t N, int64_t K, const uint16_t* L,
-                        int64_t ldl, int64_t lps, const uint16_t* R, int64_t ldr, int64_t rps,
-                        double* C, int64_t lci, int64_t lcj, void* workspace, void* stream);
+                        int64_t lps, const uint16_t* R, int64_t rps, double* C, int64_t lci,
+                        int64_t lcj, void* workspace, void* stream);
 
 #ifdef __cplusplus
 }

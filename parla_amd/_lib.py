@@ -86,11 +86,10 @@ SIGNATURES = {
     "pamd_bt_axpy": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_copy": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_layout": (c_int, [ctypes.POINTER(c_int)] * 5),
-    "pamd_split_layout": (c_int, [c_int, c_i64, c_i64, c_i64, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
-                                  ctypes.POINTER(c_i64)]),
-    "pamd_split_bf16": (c_int, [c_int, c_int, c_i64, c_i64, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp]),
-    "pamd_split_gemm_f64": (c_int, [c_int, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64,
-                                    c_i64, c_vp, c_vp]),
+    "pamd_split_layout": (c_int, [c_int, c_i64, c_i64, c_i64] + [ctypes.POINTER(c_i64)] * 4),
+    "pamd_split_bf16": (c_int, [c_int, c_int, c_i64, c_i64, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp]),
+    "pamd_split_gemm_f64": (c_int, [c_int, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp,
+                                    c_vp]),
     "pamd_bt_sjlt_csr": (c_int, [c_i64, c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_sjlt_left": (c_int, [c_i64, c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_gram": (c_int, [c_i64, c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp]),

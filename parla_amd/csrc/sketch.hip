@@ -144,57 +144,65 @@ __global__ __launch_bounds__(256) void sjlt_left_kernel(int64_t d, int64_t n, co
 constexpr int RT_ROWS = 16;   // rows of A per block
 constexpr int RT_COLS = 512;  // columns of A per LDS tile
 
-// out[r, c] = sum_{p in row c of W} vals[p] * A[r, cols[p]]
-template <int Q>  // output columns per thread = Q (k <= 256 * Q)
-__global__ __launch_bounds__(256) void sjlt_right_kernel(int64_t m, int64_t n, int64_t k,
-                                                         const int64_t* __restrict__ rowptr,
-                                                         const int32_t* __restrict__ cols,
-                                                         const double* __restrict__ vals,
-                                                         const double* __restrict__ A, int64_t lda,
-                                                         double* __restrict__ out, int64_t ldo) {
+// out[r, c] = sum_{p in row c of W} vals[p] * A[r, cols[p]]: one thread per
+// output column c (blockDim = k rounded up to 64), RT_ROWS rows of A per
+// workgroup staged through LDS RT_COLS columns at a time.  A column's CSR
+// entries are sorted by j, so the entries falling in the current tile are a
+// prefix of what remains; they are fetched eight at a time, so a thread waits
+// for the index arrays once per eight entries rather than once per entry.
+__global__ __launch_bounds__(1024) void sjlt_right_kernel(int64_t m, int64_t n, int64_t k,
+                                                          const int64_t* __restrict__ rowptr,
+                                                          const int32_t* __restrict__ cols,
+                                                          const double* __restrict__ vals,
+                                                          const double* __restrict__ A, int64_t lda,
+                                                          double* __restrict__ out, int64_t ldo) {
   __shared__ double As[RT_ROWS][RT_COLS];
   const int64_t r0 = (int64_t)blockIdx.x * RT_ROWS;
   const int nr = (int)imin64(RT_ROWS, m - r0);
-  double acc[Q][RT_ROWS];
-  int64_t ptr[Q], pend[Q];
+  const int c = threadIdx.x;
+  const bool active = c < k;
+  int64_t p = active ? rowptr[c] : 0;
+  const int64_t pend = active ? rowptr[c + 1] : 0;
+  double acc[RT_ROWS];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int64_t c = threadIdx.x + 256 * q;
-    ptr[q] = (c < k) ? rowptr[c] : 0;
-    pend[q] = (c < k) ? rowptr[c + 1] : 0;
-#pragma unroll
-    for (int rr = 0; rr < RT_ROWS; ++rr) acc[q][rr] = 0.0;
-  }
+  for (int rr = 0; rr < RT_ROWS; ++rr) acc[rr] = 0.0;
   for (int64_t j0 = 0; j0 < n; j0 += RT_COLS) {
     const int64_t jn = imin64(RT_COLS, n - j0);
     __syncthreads();
-    for (int e = threadIdx.x; e < RT_ROWS * RT_COLS; e += 256) {
+#pragma unroll 8
+    for (int e = threadIdx.x; e < RT_ROWS * RT_COLS; e += blockDim.x) {
       const int rr = e / RT_COLS, jj = e % RT_COLS;
       As[rr][jj] = (rr < nr && jj < jn) ? A[(r0 + rr) * lda + j0 + jj] : 0.0;
     }
     __syncthreads();
     const int64_t j1 = j0 + jn;
+    while (p < pend) {
+      int32_t jv[8];
+      double vv[8];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      while (ptr[q] < pend[q]) {
-        const int32_t j = cols[ptr[q]];
-        if (j >= j1) break;
-        const double v = vals[ptr[q]];
-        const int jj = j - (int)j0;
-#pragma unroll
-        for (int rr = 0; rr < RT_ROWS; ++rr) acc[q][rr] += v * As[rr][jj];
-        ++ptr[q];
+      for (int e = 0; e < 8; ++e) {
+        const bool in = p + e < pend;
+        jv[e] = in ? cols[p + e] : INT32_MAX;
+        vv[e] = in ? vals[p + e] : 0.0;
       }
+      int used = 0;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        if (jv[e] < j1) {  // a prefix of the eight (sorted entries)
+          const int jl = jv[e] - (int)j0;
+#pragma unroll
+          for (int rr = 0; rr < RT_ROWS; ++rr) acc[rr] += vv[e] * As[rr][jl];
+          used = e + 1;
+        }
+      }
+      p += used;
+      if (used < 8) break;  // the rest of this column lies in later tiles
     }
   }
+  if (active)
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int64_t c = threadIdx.x + 256 * q;
-    if (c < k)
-#pragma unroll
-      for (int rr = 0; rr < RT_ROWS; ++rr)
-        if (rr < nr) out[(r0 + rr) * ldo + c] = acc[q][rr];
-  }
+    for (int rr = 0; rr < RT_ROWS; ++rr)
+      if (rr < nr) out[(r0 + rr) * ldo + c] = acc[rr];
 }
 
 __global__ void transpose_kernel(int64_t rows, int64_t cols, const double* __restrict__ A, int64_t lda,
@@ -313,12 +321,8 @@ extern "C" int pamd_sjlt_right_f64(int64_t m, int64_t n, int64_t k, const int64_
   if (m == 0 || k == 0) return PAMD_OK;
   hipStream_t st = (hipStream_t)stream;
   const unsigned nb = (unsigned)pamd_cdiv(m, RT_ROWS);
-  if (k <= 256)
-    sjlt_right_kernel<1><<<nb, 256, 0, st>>>(m, n, k, rowptr, cols, vals, A, lda, out, ldo);
-  else if (k <= 512)
-    sjlt_right_kernel<2><<<nb, 256, 0, st>>>(m, n, k, rowptr, cols, vals, A, lda, out, ldo);
-  else
-    sjlt_right_kernel<4><<<nb, 256, 0, st>>>(m, n, k, rowptr, cols, vals, A, lda, out, ldo);
+  const unsigned threads = (unsigned)(pamd_cdiv(k, 64) * 64);
+  sjlt_right_kernel<<<nb, threads, 0, st>>>(m, n, k, rowptr, cols, vals, A, lda, out, ldo);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

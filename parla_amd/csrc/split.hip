@@ -6,22 +6,29 @@
 //   P = 3: 24 bits (the FP32 significand), six products.
 // The dropped products are below 2^-(8P) of |L| |R| elementwise.
 //
-// Layout: both operands K-contiguous ("TN"): L_p is M x K row-major, R_q is
-// N x K row-major (the transpose of the K x N right factor), K zero-padded
-// to a multiple of 64 and R's rows zero-padded to a multiple of 288.  The
-// split kernels produce either orientation from a row-major FP64 matrix, so
-// A (m x n) serves A S (L = A) and A^T Y (L = A^T) and each tall k-column
-// factor is split straight into its transpose.
+// Piece layout ("blocked"): both operands K-contiguous ("TN"), L_p with M
+// rows and R_q with N rows (the transpose of the K x N right factor), stored
+// as 1 KiB blocks of 16 rows x 32 K: element (i, k) of a piece with kp
+// (multiple of 64) columns sits at
+//   ((i / 16) (kp / 32) + k / 32) 512 + (i % 16) 32 + 8 (c ^ ((i % 16) / 4)) + k % 8,
+// c = (k % 32) / 8 the 16-byte chunk of the row.  A workgroup's K step is
+// then whole 1 KiB blocks (full-burst loads), and the chunk swizzle makes the
+// MFMA operand reads from LDS conflict-free without padding.  Rows beyond
+// the matrix and columns beyond K are zero.  The split kernels write this
+// layout from a row-major FP64 matrix, in either orientation, so A (m x n)
+// serves A S (L = A) and A^T Y (L = A^T) and each tall k-column factor is
+// split straight into its transpose.
 //
 // GEMM: one 256-thread workgroup per 128 x TN output tile and K range; the
 // four wavefronts split the tile 2 x 2 (P = 2, TN = 288) or 4 x 1 (P = 3,
-// TN = 144).  K advances 32 at a time through double-buffered LDS with the
-// next step's global loads in registers.  Skinny products (N ~ 300) have few
-// tiles, so K is split until there are at least 256 workgroups; the splits'
-// FP32 partials are summed in split order in FP64 by a second kernel, which
-// also writes C in either orientation.  Tiles are assigned XCD-aware: the
-// workgroups of one XCD share their K range and column tile, so the right
-// operand's slice is fetched into that XCD's L2 once.
+// TN = 144).  K advances 32 at a time through double-buffered LDS, with the
+// global loads two steps ahead in two register banks.  Skinny products
+// (N ~ 300) have few tiles, so K is split until there are at least 256
+// workgroups; the splits' FP32 partials are summed in split order in FP64 by
+// a second kernel, which also writes C in either orientation.  Tiles are
+// assigned XCD-aware: the workgroups of one XCD share their K range and
+// column tile, so the right operand's slice is fetched into that XCD's L2
+// once.
 #include "common.h"
 
 typedef __bf16 sg_bf16x8 __attribute__((ext_vector_type(8)));
@@ -31,7 +38,6 @@ namespace {
 
 constexpr int SG_TM = 128;
 constexpr int SG_BK = 32;
-constexpr int SG_LDR = SG_BK + 8;  // LDS row stride in BF16: 80 bytes, conflict-free 16-byte reads
 constexpr int SG_THREADS = 256;
 constexpr int SG_KALIGN = 64;
 constexpr int SG_NALIGN = 288;
@@ -63,37 +69,45 @@ __device__ __forceinline__ uint4 sg_pack(const uint16_t (&v)[8]) {
   return w;
 }
 
-// pieces[p][i][k] of X (rows x cols, row-major) for i < rows_out, k < kp;
-// zero outside X.  Thread: 8 consecutive k of one row.
-template <int P>
-__global__ __launch_bounds__(256) void sg_split_rows_kernel(int64_t rows, int64_t cols, const double* __restrict__ X,
-                                                            int64_t ldx, int64_t rows_out, int64_t kp,
-                                                            uint16_t* __restrict__ pieces, int64_t ldk,
-                                                            int64_t pstride) {
-  const int64_t k0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 8;
-  if (k0 >= kp) return;
-  for (int64_t i = blockIdx.y; i < rows_out; i += gridDim.y) {
-    uint16_t h[P][8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int64_t j = k0 + e;
-      const double x = (i < rows && j < cols) ? X[i * ldx + j] : 0.0;
-      uint16_t t[P];
-      sg_split<P>(x, t);
-#pragma unroll
-      for (int p = 0; p < P; ++p) h[p][e] = t[p];
-    }
-#pragma unroll
-    for (int p = 0; p < P; ++p) *reinterpret_cast<uint4*>(pieces + p * pstride + i * ldk + k0) = sg_pack(h[p]);
-  }
+// offset (in BF16 elements) of element (i, k) in a blocked piece array
+__device__ __forceinline__ int64_t sg_off(int64_t i, int64_t k, int64_t kp) {
+  const int r = (int)(i & 15), c = (int)((k & 31) >> 3);
+  return ((i >> 4) * (kp >> 5) + (k >> 5)) * 512 + r * 32 + ((c ^ (r >> 2)) << 3) + (k & 7);
 }
 
-// pieces[p][j][i] of X^T for j < rows_out, i < kp (kp a multiple of 64):
-// 64 x 64 tiles transposed through LDS, coalesced on both sides
+// blocked pieces of X (rows x cols, row-major): one wavefront per 16 x 32
+// block, lane l = row l / 4, chunk l % 4 (eight consecutive k)
+template <int P>
+__global__ __launch_bounds__(256) void sg_split_rows_kernel(int64_t rows, int64_t cols, const double* __restrict__ X,
+                                                            int64_t ldx, int64_t nblk, int64_t kb_per_row,
+                                                            uint16_t* __restrict__ pieces, int64_t pstride) {
+  const int64_t blk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (blk >= nblk) return;
+  const int lane = threadIdx.x & 63, r = lane >> 2, c = lane & 3;
+  const int64_t i = (blk / kb_per_row) * 16 + r;
+  const int64_t k0 = (blk % kb_per_row) * 32 + 8 * c;
+  uint16_t h[P][8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int64_t j = k0 + e;
+    const double x = (i < rows && j < cols) ? X[i * ldx + j] : 0.0;
+    uint16_t t[P];
+    sg_split<P>(x, t);
+#pragma unroll
+    for (int p = 0; p < P; ++p) h[p][e] = t[p];
+  }
+  const int64_t off = blk * 512 + r * 32 + ((c ^ (r >> 2)) << 3);
+#pragma unroll
+  for (int p = 0; p < P; ++p) *reinterpret_cast<uint4*>(pieces + p * pstride + off) = sg_pack(h[p]);
+}
+
+// blocked pieces of X^T (row j of the piece array = column j of X, K = X's
+// rows): a 64 x 64 tile of X transposed through LDS, then each wavefront
+// writes two 16 x 32 blocks
 template <int P>
 __global__ __launch_bounds__(256) void sg_split_t_kernel(int64_t rows, int64_t cols, const double* __restrict__ X,
-                                                         int64_t ldx, int64_t rows_out, uint16_t* __restrict__ pieces,
-                                                         int64_t ldk, int64_t pstride) {
+                                                         int64_t ldx, int64_t rows_out, int64_t kp,
+                                                         uint16_t* __restrict__ pieces, int64_t pstride) {
   __shared__ double T[64][65];
   const int tid = threadIdx.x;
   const int64_t i0 = (int64_t)blockIdx.x * 64, j0 = (int64_t)blockIdx.y * 64;
@@ -104,87 +118,80 @@ __global__ __launch_bounds__(256) void sg_split_t_kernel(int64_t rows, int64_t c
     T[ii][jj] = (i < rows && j < cols) ? X[i * ldx + j] : 0.0;
   }
   __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6, r = lane >> 2, c = lane & 3;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    const int ch = tid + 256 * h;
-    const int jj = ch / 8, ii = (ch % 8) * 8;
-    const int64_t j = j0 + jj;
+    const int b = wave * 2 + h;  // block (jr, kr) of the tile: 4 row blocks x 2 K blocks
+    const int jr = b >> 1, kr = b & 1;
+    const int64_t j = j0 + jr * 16 + r;
     if (j >= rows_out) continue;
     uint16_t v[P][8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       uint16_t t[P];
-      sg_split<P>(T[ii + e][jj], t);
+      sg_split<P>(T[kr * 32 + 8 * c + e][jr * 16 + r], t);
 #pragma unroll
       for (int p = 0; p < P; ++p) v[p][e] = t[p];
     }
+    const int64_t off = sg_off(j, i0 + kr * 32 + 8 * c, kp);
 #pragma unroll
-    for (int p = 0; p < P; ++p) *reinterpret_cast<uint4*>(pieces + p * pstride + j * ldk + i0 + ii) = sg_pack(v[p]);
+    for (int p = 0; p < P; ++p) *reinterpret_cast<uint4*>(pieces + p * pstride + off) = sg_pack(v[p]);
   }
 }
 
-// ws[ks][i][j] (FP32) = sum over p + q < P of L_p[i, K_ks] R_q[j, K_ks]^T
+typedef __attribute__((address_space(3))) void sg_lds_void;
+typedef const __attribute__((address_space(1))) void sg_glb_void;
+
+// ws[ks][i][j] (FP32) = sum over p + q < P of L_p[i, K_ks] R_q[j, K_ks]^T.
+// K steps stream through three LDS stages by direct global -> LDS loads
+// (global_load_lds_dwordx4: a wavefront moves one 1 KiB block per
+// instruction, no registers involved); the loads for step s + 2 are issued
+// during step s.  The stages are separate arrays and the loop is unrolled by
+// three, so every access names its stage at compile time.
 template <int P, int WM, int WN, int MB, int NB>
-__global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t M, int64_t kp, int64_t kc, int mtiles,
-                                                             int ntiles, int nsplit,
-                                                             const uint16_t* __restrict__ L, int64_t ldl,
+__global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t kc, int mtiles, int ntiles,
+                                                             int nsplit, const uint16_t* __restrict__ L,
                                                              int64_t lps, const uint16_t* __restrict__ R,
-                                                             int64_t ldr, int64_t rps, float* __restrict__ ws,
+                                                             int64_t rps, int64_t M, float* __restrict__ ws,
                                                              int64_t npad) {
   constexpr int TM = WM * MB * 16, TN = WN * NB * 16;
   static_assert(TM == SG_TM && WM * WN == 4, "four wavefronts on a 128-row tile");
-  constexpr int LCH = P * TM * 4, RCH = P * TN * 4;  // 16-byte chunks per K step
-  constexpr int LPT = (LCH + SG_THREADS - 1) / SG_THREADS, RPT = (RCH + SG_THREADS - 1) / SG_THREADS;
-  __shared__ __attribute__((aligned(16))) uint16_t Ls[2][P][TM][SG_LDR];
-  __shared__ __attribute__((aligned(16))) uint16_t Rs[2][P][TN][SG_LDR];
+  constexpr int LB = TM / 16, RB = TN / 16;  // 1 KiB blocks per piece per K step
+  constexpr int NBLK = P * (LB + RB);        // blocks per K step: L pieces, then R pieces
+  constexpr int PERW = NBLK / 4;             // blocks every wavefront loads per step (at least)
+  __shared__ uint4 S0[NBLK][64], S1[NBLK][64], S2[NBLK][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   const int lr = lane & 15, lk = lane >> 4;
+  const int frag = lr * 4 + (lk ^ (lr >> 2));  // this lane's swizzled 16-byte chunk in a block
   const uint32_t total = (uint32_t)mtiles * ntiles * nsplit;
   const uint32_t t = xcd_tile(blockIdx.x, total);
   const int ks = (int)(t / ((uint32_t)mtiles * ntiles));
   const uint32_t rem = t % ((uint32_t)mtiles * ntiles);
   const int nt = (int)(rem / mtiles), mt = (int)(rem % mtiles);
-  const int64_t i0 = (int64_t)mt * TM, j0 = (int64_t)nt * TN;
-  const int64_t kbeg = (int64_t)ks * kc;
-  const int64_t kend = imin64(kp, kbeg + kc);
-  const int nsteps = (int)((kend - kbeg) / SG_BK);
+  const int64_t kbn = kp >> 5;  // K blocks per row block
+  const int64_t kb0 = ((int64_t)ks * kc) >> 5;
+  const int nsteps = (int)((imin64(kp, (int64_t)(ks + 1) * kc) - (int64_t)ks * kc) >> 5);
+  const uint4* Lg = reinterpret_cast<const uint4*>(L) + lane;
+  const uint4* Rg = reinterpret_cast<const uint4*>(R) + lane;
+  const int64_t lps4 = lps >> 3, rps4 = rps >> 3;
 
-  uint4 rl[LPT], rr[RPT];
-  auto load = [&](int64_t k0) {
+  // wavefront w loads blocks w, w + 4, ... of step `step` into stage `dst`
+  auto issue = [&](int step, uint4 (*dst)[64]) {
+    const int64_t kb = kb0 + step;
 #pragma unroll
-    for (int h = 0; h < LPT; ++h) {
-      const int ch = tid + SG_THREADS * h;
-      if (ch < LCH) {
-        const int p = ch / (TM * 4), r = (ch / 4) % TM, c = ch % 4;
-        const int64_t gi = i0 + r;
-        rl[h] = (gi < M) ? *reinterpret_cast<const uint4*>(L + p * lps + gi * ldl + k0 + 8 * c) : uint4{0, 0, 0, 0};
-      }
-    }
-#pragma unroll
-    for (int h = 0; h < RPT; ++h) {
-      const int ch = tid + SG_THREADS * h;
-      if (ch < RCH) {
-        const int q = ch / (TN * 4), r = (ch / 4) % TN, c = ch % 4;
-        rr[h] = *reinterpret_cast<const uint4*>(R + q * rps + (j0 + r) * ldr + k0 + 8 * c);
-      }
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int h = 0; h < LPT; ++h) {
-      const int ch = tid + SG_THREADS * h;
-      if (ch < LCH) {
-        const int p = ch / (TM * 4), r = (ch / 4) % TM, c = ch % 4;
-        *reinterpret_cast<uint4*>(&Ls[buf][p][r][8 * c]) = rl[h];
-      }
-    }
-#pragma unroll
-    for (int h = 0; h < RPT; ++h) {
-      const int ch = tid + SG_THREADS * h;
-      if (ch < RCH) {
-        const int q = ch / (TN * 4), r = (ch / 4) % TN, c = ch % 4;
-        *reinterpret_cast<uint4*>(&Rs[buf][q][r][8 * c]) = rr[h];
+    for (int i = 0; i < (NBLK + 3) / 4; ++i) {
+      const int b = wave + 4 * i;
+      if (b < NBLK) {
+        const uint4* src;
+        if (b < P * LB) {
+          const int p = b / LB, rb = b % LB;
+          src = Lg + p * lps4 + (((int64_t)mt * LB + rb) * kbn + kb) * 64;
+        } else {
+          const int q = (b - P * LB) / RB, rb = (b - P * LB) % RB;
+          src = Rg + q * rps4 + (((int64_t)nt * RB + rb) * kbn + kb) * 64;
+        }
+        __builtin_amdgcn_global_load_lds((sg_glb_void*)src, (sg_lds_void*)&dst[b][0], 16, 0, 0);
       }
     }
   };
@@ -195,26 +202,28 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t M, int64_t 
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = sg_f4{0.f, 0.f, 0.f, 0.f};
 
-  if (nsteps > 0) {
-    load(kbeg);
-    store(0);
-  }
-  __syncthreads();
-  if (nsteps > 1) load(kbeg + SG_BK);
-  for (int s = 0; s < nsteps; ++s) {
-    const int buf = s & 1;
-    sg_bf16x8 a[P][MB];
+  // one K step: wait for this wavefront's loads of the step (the loads of
+  // the next step may stay in flight), barrier (every wavefront's loads are
+  // in and the stage read two steps ago is free), read the fragments, issue
+  // the loads two steps ahead, then the MFMAs
+  auto step = [&](int s, uint4 (*cur)[64], uint4 (*ahead)[64]) {
+    __builtin_amdgcn_s_waitcnt((PERW & 15) | ((PERW >> 4) << 14) | 0x3F70);  // vmcnt(PERW)
+    asm volatile("s_barrier" ::: "memory");
+    sg_bf16x8 a[P][MB], b[P][NB];
 #pragma unroll
     for (int p = 0; p < P; ++p)
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb)
-        a[p][mb] = *reinterpret_cast<const sg_bf16x8*>(&Ls[buf][p][wm * MB * 16 + mb * 16 + lr][8 * lk]);
+        a[p][mb] = *reinterpret_cast<const sg_bf16x8*>(&cur[p * LB + wm * MB + mb][frag]);
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-      sg_bf16x8 b[P];
+    for (int q = 0; q < P; ++q)
 #pragma unroll
-      for (int q = 0; q < P; ++q)
-        b[q] = *reinterpret_cast<const sg_bf16x8*>(&Rs[buf][q][wn * NB * 16 + nb * 16 + lr][8 * lk]);
+      for (int nb = 0; nb < NB; ++nb)
+        b[q][nb] = *reinterpret_cast<const sg_bf16x8*>(&cur[P * LB + q * RB + wn * NB + nb][frag]);
+    asm volatile("" ::: "memory");
+    if (s + 2 < nsteps) issue(s + 2, ahead);
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
       for (int p = 0; p < P; ++p)
 #pragma unroll
@@ -222,17 +231,20 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t M, int64_t 
           if (p + q >= P) continue;
 #pragma unroll
           for (int mb = 0; mb < MB; ++mb)
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p][mb], b[q], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p][mb], b[q][nb], acc[mb][nb], 0, 0, 0);
         }
-    }
-    if (s + 1 < nsteps) {
-      store(buf ^ 1);
-      if (s + 2 < nsteps) load(kbeg + (int64_t)(s + 2) * SG_BK);
-    }
-    __syncthreads();
+  };
+
+  if (nsteps > 0) issue(0, S0);
+  if (nsteps > 1) issue(1, S1);
+  for (int s = 0; s < nsteps; s += 3) {
+    step(s, S0, S2);
+    if (s + 1 < nsteps) step(s + 1, S1, S0);
+    if (s + 2 < nsteps) step(s + 2, S2, S1);
   }
   // D layout of v_mfma_f32_16x16x32_bf16: lane l, item r -> row 4 (l / 16) + r, column l % 16
   float* w = ws + (int64_t)ks * M * npad;
+  const int64_t i0 = (int64_t)mt * TM, j0 = (int64_t)nt * TN;
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
@@ -276,15 +288,16 @@ __global__ __launch_bounds__(256) void sg_reduce_kernel(int64_t M, int64_t N, in
 
 struct SgShape {
   int mtiles, ntiles, nsplit;
-  int64_t kp, kc, npad;
+  int64_t kp, kc, mpad, npad;
 };
 
 SgShape sg_shape(int P, int64_t M, int64_t N, int64_t K) {
   SgShape s;
   const int tn = (P == 2) ? 288 : 144;
   s.kp = pamd_cdiv(imax64(K, 1), SG_KALIGN) * SG_KALIGN;
+  s.mpad = pamd_cdiv(imax64(M, 1), SG_TM) * SG_TM;
   s.npad = pamd_cdiv(imax64(N, 1), SG_NALIGN) * SG_NALIGN;
-  s.mtiles = (int)pamd_cdiv(imax64(M, 1), SG_TM);
+  s.mtiles = (int)(s.mpad / SG_TM);
   s.ntiles = (int)pamd_cdiv(imax64(N, 1), tn);
   const int64_t tiles = (int64_t)s.mtiles * s.ntiles;
   const int64_t ksteps = s.kp / SG_BK;
@@ -299,25 +312,26 @@ SgShape sg_shape(int P, int64_t M, int64_t N, int64_t K) {
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-extern "C" int pamd_split_layout(int npieces, int64_t M, int64_t N, int64_t K, int64_t* kp, int64_t* npad,
-                                 int64_t* workspace_bytes) {
+extern "C" int pamd_split_layout(int npieces, int64_t M, int64_t N, int64_t K, int64_t* kp, int64_t* mpad,
+                                 int64_t* npad, int64_t* workspace_bytes) {
   PAMD_ARG_CHECK(npieces == 2 || npieces == 3, 1);
   PAMD_ARG_CHECK(M >= 0, 2);
   PAMD_ARG_CHECK(N >= 0, 3);
   PAMD_ARG_CHECK(K >= 0, 4);
   PAMD_ARG_CHECK(kp != nullptr, 5);
-  PAMD_ARG_CHECK(npad != nullptr, 6);
-  PAMD_ARG_CHECK(workspace_bytes != nullptr, 7);
+  PAMD_ARG_CHECK(mpad != nullptr, 6);
+  PAMD_ARG_CHECK(npad != nullptr, 7);
+  PAMD_ARG_CHECK(workspace_bytes != nullptr, 8);
   const SgShape s = sg_shape(npieces, M, N, K);
   *kp = s.kp;
+  *mpad = s.mpad;
   *npad = s.npad;
   *workspace_bytes = (int64_t)s.nsplit * imax64(M, 1) * s.npad * (int64_t)sizeof(float);
   return PAMD_OK;
 }
 
 extern "C" int pamd_split_bf16(int transpose, int npieces, int64_t rows, int64_t cols, const double* X, int64_t ldx,
-                               int64_t rows_out, int64_t kp, uint16_t* pieces, int64_t ldk, int64_t pstride,
-                               void* stream) {
+                               int64_t rows_out, int64_t kp, uint16_t* pieces, int64_t pstride, void* stream) {
   PAMD_ARG_CHECK(transpose == 0 || transpose == 1, 1);
   PAMD_ARG_CHECK(npieces == 2 || npieces == 3, 2);
   PAMD_ARG_CHECK(rows >= 0, 3);
@@ -325,34 +339,35 @@ extern "C" int pamd_split_bf16(int transpose, int npieces, int64_t rows, int64_t
   PAMD_ARG_CHECK(X != nullptr || rows * cols == 0, 5);
   PAMD_ARG_CHECK(ldx >= cols && ldx >= 1, 6);
   // rows_out: rows of the piece array (X's rows, or its columns when transposed)
-  PAMD_ARG_CHECK(rows_out >= (transpose ? cols : rows), 7);
+  PAMD_ARG_CHECK(rows_out >= (transpose ? cols : rows) && rows_out % 16 == 0, 7);
   PAMD_ARG_CHECK(kp % SG_KALIGN == 0 && kp >= (transpose ? rows : cols), 8);
   PAMD_ARG_CHECK(pieces != nullptr, 9);
-  PAMD_ARG_CHECK(ldk >= kp && ldk % 8 == 0, 10);
-  PAMD_ARG_CHECK(pstride >= ldk * rows_out && pstride % 8 == 0, 11);
+  PAMD_ARG_CHECK(pstride >= rows_out * kp && pstride % 8 == 0, 10);
   if (rows_out == 0 || kp == 0) return PAMD_OK;
   hipStream_t st = (hipStream_t)stream;
   if (!transpose) {
-    dim3 grid((unsigned)pamd_cdiv(kp / 8, 256), (unsigned)imin64(rows_out, 65535));
+    const int64_t kbn = kp / 32, nblk = (rows_out / 16) * kbn;
+    PAMD_ARG_CHECK(pamd_cdiv(nblk, 4) < (1LL << 31), 7);
+    const unsigned grid = (unsigned)pamd_cdiv(nblk, 4);
     if (npieces == 2)
-      sg_split_rows_kernel<2><<<grid, 256, 0, st>>>(rows, cols, X, ldx, rows_out, kp, pieces, ldk, pstride);
+      sg_split_rows_kernel<2><<<grid, 256, 0, st>>>(rows, cols, X, ldx, nblk, kbn, pieces, pstride);
     else
-      sg_split_rows_kernel<3><<<grid, 256, 0, st>>>(rows, cols, X, ldx, rows_out, kp, pieces, ldk, pstride);
+      sg_split_rows_kernel<3><<<grid, 256, 0, st>>>(rows, cols, X, ldx, nblk, kbn, pieces, pstride);
   } else {
     PAMD_ARG_CHECK(pamd_cdiv(rows_out, 64) <= 65535, 7);
     dim3 grid((unsigned)(kp / 64), (unsigned)pamd_cdiv(rows_out, 64));
     if (npieces == 2)
-      sg_split_t_kernel<2><<<grid, 256, 0, st>>>(rows, cols, X, ldx, rows_out, pieces, ldk, pstride);
+      sg_split_t_kernel<2><<<grid, 256, 0, st>>>(rows, cols, X, ldx, rows_out, kp, pieces, pstride);
     else
-      sg_split_t_kernel<3><<<grid, 256, 0, st>>>(rows, cols, X, ldx, rows_out, pieces, ldk, pstride);
+      sg_split_t_kernel<3><<<grid, 256, 0, st>>>(rows, cols, X, ldx, rows_out, kp, pieces, pstride);
   }
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }
 
-extern "C" int pamd_split_gemm_f64(int npieces, int64_t M, int64_t N, int64_t K, const uint16_t* L, int64_t ldl,
-                                   int64_t lps, const uint16_t* R, int64_t ldr, int64_t rps, double* C, int64_t lci,
-                                   int64_t lcj, void* workspace, void* stream) {
+extern "C" int pamd_split_gemm_f64(int npieces, int64_t M, int64_t N, int64_t K, const uint16_t* L, int64_t lps,
+                                   const uint16_t* R, int64_t rps, double* C, int64_t lci, int64_t lcj,
+                                   void* workspace, void* stream) {
   PAMD_ARG_CHECK(npieces == 2 || npieces == 3, 1);
   PAMD_ARG_CHECK(M >= 0, 2);
   PAMD_ARG_CHECK(N >= 0, 3);
@@ -360,24 +375,22 @@ extern "C" int pamd_split_gemm_f64(int npieces, int64_t M, int64_t N, int64_t K,
   if (M == 0 || N == 0) return PAMD_OK;
   const SgShape s = sg_shape(npieces, M, N, K);
   PAMD_ARG_CHECK(L != nullptr, 5);
-  PAMD_ARG_CHECK(ldl >= s.kp && ldl % 8 == 0, 6);
-  PAMD_ARG_CHECK(lps >= ldl * M && lps % 8 == 0, 7);
-  PAMD_ARG_CHECK(R != nullptr, 8);
-  PAMD_ARG_CHECK(ldr >= s.kp && ldr % 8 == 0, 9);
-  PAMD_ARG_CHECK(rps >= ldr * s.npad && rps % 8 == 0, 10);
-  PAMD_ARG_CHECK(C != nullptr, 11);
-  PAMD_ARG_CHECK((lcj == 1 && lci >= N) || (lci == 1 && lcj >= M), 12);
-  PAMD_ARG_CHECK(workspace != nullptr, 14);
+  PAMD_ARG_CHECK(lps >= s.mpad * s.kp && lps % 8 == 0, 6);
+  PAMD_ARG_CHECK(R != nullptr, 7);
+  PAMD_ARG_CHECK(rps >= s.npad * s.kp && rps % 8 == 0, 8);
+  PAMD_ARG_CHECK(C != nullptr, 9);
+  PAMD_ARG_CHECK((lcj == 1 && lci >= N) || (lci == 1 && lcj >= M), 10);
+  PAMD_ARG_CHECK(workspace != nullptr, 12);
   hipStream_t st = (hipStream_t)stream;
   const int64_t total = (int64_t)s.mtiles * s.ntiles * s.nsplit;
   PAMD_ARG_CHECK(total < (1LL << 31), 2);
   float* ws = (float*)workspace;
   if (npieces == 2)
     sg_gemm_kernel<2, 2, 2, 4, 9><<<(unsigned)total, SG_THREADS, 0, st>>>(
-        M, s.kp, s.kc, s.mtiles, s.ntiles, s.nsplit, L, ldl, lps, R, ldr, rps, ws, s.npad);
+        s.kp, s.kc, s.mtiles, s.ntiles, s.nsplit, L, lps, R, rps, M, ws, s.npad);
   else
     sg_gemm_kernel<3, 4, 1, 2, 9><<<(unsigned)total, SG_THREADS, 0, st>>>(
-        M, s.kp, s.kc, s.mtiles, s.ntiles, s.nsplit, L, ldl, lps, R, ldr, rps, ws, s.npad);
+        s.kp, s.kc, s.mtiles, s.ntiles, s.nsplit, L, lps, R, rps, M, ws, s.npad);
   PAMD_LAUNCH_CHECK();
   dim3 rgrid((unsigned)pamd_cdiv(M, 64), (unsigned)pamd_cdiv(N, 64));
   sg_reduce_kernel<<<rgrid, 256, 0, st>>>(M, N, s.npad, s.nsplit, ws, C, lci, lcj);

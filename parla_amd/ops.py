@@ -515,17 +515,17 @@ def trsm_rec_(X, R):
 # Split-precision GEMM: FP64 operands as BF16 pieces on BF16 MFMA
 # ---------------------------------------------------------------------------
 def split_layout(npieces, M, N, K):
-    """(kp, npad, workspace bytes) of the split GEMM C (M x N) = L R^T, K
-    reduced: pieces need kp >= K columns and R needs npad >= N rows."""
-    kp, npad, ws = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-    check(lib().pamd_split_layout(npieces, M, N, K, ctypes.byref(kp), ctypes.byref(npad), ctypes.byref(ws)),
-          "pamd_split_layout")
-    return kp.value, npad.value, ws.value
+    """(kp, mpad, npad, workspace bytes) of the split GEMM C (M x N) = L R^T
+    with K reduced: piece arrays have kp columns, L's mpad rows, R's npad."""
+    v = [ctypes.c_int64() for _ in range(4)]
+    check(lib().pamd_split_layout(npieces, M, N, K, *[ctypes.byref(x) for x in v]), "pamd_split_layout")
+    return tuple(x.value for x in v)
 
 
 class SplitPieces:
-    """BF16 pieces (npieces x rows_out x kp, uint16 bits) of a K-contiguous
-    operand: `rows` logical rows, K logical columns."""
+    """BF16 pieces (npieces x rows_out x kp, uint16 bits, 16 x 32 blocked
+    layout) of a K-contiguous operand with `rows` logical rows and K
+    logical columns."""
 
     def __init__(self, pieces, rows, K):
         self.pieces = pieces
@@ -538,26 +538,28 @@ class SplitPieces:
 
 
 def split_bf16(X, npieces, transpose=False, rows_out=None, kp=None):
-    """Pieces of X (row-major FP64, rows x cols) or, with transpose, of X^T."""
+    """Pieces of X (row-major FP64, rows x cols) or, with transpose, of X^T;
+    rows_out defaults to the row count rounded up to 128 (a left operand)."""
     Xc = X if X.stride(-1) == 1 else X.contiguous()
     rows, cols = Xc.shape
     out_rows, K = (cols, rows) if transpose else (rows, cols)
     kp = kp or _round_up(max(K, 1), 64)
-    rows_out = rows_out or out_rows
+    rows_out = rows_out or _round_up(max(out_rows, 1), 128)
     P = torch.empty((npieces, rows_out, kp), dtype=torch.uint16, device=X.device)
     check(lib().pamd_split_bf16(int(transpose), npieces, rows, cols, _p(Xc), max(Xc.stride(0), cols, 1), rows_out,
-                                kp, _p(P), kp, rows_out * kp, stream(X.device)), "pamd_split_bf16")
+                                kp, _p(P), rows_out * kp, stream(X.device)), "pamd_split_bf16")
     return SplitPieces(P, out_rows, K)
 
 
 def split_gemm(L, R, out=None, transpose_out=False):
     """C = sum_{p+q<P} L_p R_q^T (M x N, FP64) from SplitPieces L (M x K) and
-    R (N x K, its piece array padded to the layout's npad rows); with
+    R (N x K, its piece array with the layout's npad rows); with
     transpose_out the result is written as C^T (N x M)."""
     npieces = L.npieces
     M, N, K = L.rows, R.rows, L.K
-    kp, npad, wsb = split_layout(npieces, M, N, K)
-    if L.pieces.shape[2] < kp or R.pieces.shape[2] < kp or R.pieces.shape[1] < npad or R.npieces != npieces:
+    kp, mpad, npad, wsb = split_layout(npieces, M, N, K)
+    if (L.pieces.shape[2] != kp or R.pieces.shape[2] != kp or L.pieces.shape[1] < mpad
+            or R.pieces.shape[1] < npad or R.npieces != npieces or R.K != K):
         raise ValueError("split_gemm: piece arrays do not match the layout")
     dev = L.pieces.device
     if out is None:
@@ -565,10 +567,31 @@ def split_gemm(L, R, out=None, transpose_out=False):
     lci, lcj = (1, out.stride(0)) if transpose_out else (out.stride(0), 1)
     ws = workspace(wsb, dev)
     Lp, Rp = L.pieces, R.pieces
-    check(lib().pamd_split_gemm_f64(npieces, M, N, K, _p(Lp), Lp.shape[2], Lp.shape[1] * Lp.shape[2], _p(Rp),
-                                    Rp.shape[2], Rp.shape[1] * Rp.shape[2], _p(out), lci, lcj, _p(ws),
-                                    stream(dev)), "pamd_split_gemm_f64")
+    check(lib().pamd_split_gemm_f64(npieces, M, N, K, _p(Lp), Lp.shape[1] * kp, _p(Rp), Rp.shape[1] * kp,
+                                    _p(out), lci, lcj, _p(ws), stream(dev)), "pamd_split_gemm_f64")
     return out
+
+
+def block_pieces(a):
+    """Blocked piece array (uint16 tensor) from a row-major numpy one
+    (npieces x rows x kp, rows a multiple of 16, kp of 64)."""
+    P, R, kp = a.shape
+    b = np.ascontiguousarray(a).reshape(P, R // 16, 16, kp // 32, 4, 8).transpose(0, 1, 3, 2, 4, 5)
+    r = np.arange(16)[:, None]
+    logical = np.arange(4)[None, :] ^ (r >> 2)  # the logical chunk stored at physical position c in row r
+    return torch.from_numpy(np.ascontiguousarray(b[:, :, :, r, logical, :]).reshape(P, R, kp))
+
+
+def unblock_pieces(pieces):
+    """Row-major view (numpy, npieces x rows_out x kp) of a blocked piece
+    array: undoes the 16 x 32 blocking and the chunk swizzle."""
+    a = pieces.cpu().numpy()
+    P, R, kp = a.shape
+    b = a.reshape(P, R // 16, kp // 32, 16, 4, 8)
+    r = np.arange(16)[:, None]
+    phys = np.arange(4)[None, :] ^ (r >> 2)  # physical chunk of logical chunk c in row r
+    b = b[:, :, :, r, phys, :]               # (P, R/16, kp/32, 16, 4, 8) in logical order
+    return b.transpose(0, 1, 3, 2, 4, 5).reshape(P, R, kp)
 
 
 def sjlt_draw(n_short, n_long, k, seed, dev):

@@ -52,7 +52,7 @@ class SplitOperator:
         return self._parts[2] if self._transposed else self._parts[1]
 
     def _right_of(self, X, K, N, transpose):
-        kp, npad, _ = ops.split_layout(self.npieces, 1, N, K)
+        kp, _, npad, _ = ops.split_layout(self.npieces, 1, N, K)
         return ops.split_bf16(X, self.npieces, transpose=transpose, rows_out=npad, kp=kp)
 
     def __matmul__(self, X):

@@ -32,14 +32,14 @@ def test_split_kernels_bit_exact(ops, P, rows, cols):
     X = rng.standard_normal((rows, cols)) * np.exp(rng.uniform(-10, 10, (rows, cols)))
     Xd = torch.from_numpy(X).cuda()
     ref = orc.bf16_pieces(X, P)
-    a = ops.split_bf16(Xd, P, rows_out=rows + 5)
-    got = a.pieces.cpu().numpy()
+    a = ops.split_bf16(Xd, P, rows_out=(rows + 5 + 15) // 16 * 16)
+    got = ops.unblock_pieces(a.pieces)
     kp = got.shape[2]
     assert kp % 64 == 0 and kp >= cols
     assert np.array_equal(got[:, :rows, :cols], ref)
     assert not got[:, rows:, :].any() and not got[:, :, cols:].any()
-    b = ops.split_bf16(Xd, P, transpose=True, rows_out=cols + 3)
-    got = b.pieces.cpu().numpy()
+    b = ops.split_bf16(Xd, P, transpose=True, rows_out=(cols + 3 + 15) // 16 * 16)
+    got = ops.unblock_pieces(b.pieces)
     assert np.array_equal(got[:, :cols, :rows], ref.transpose(0, 2, 1))
     assert not got[:, cols:, :].any() and not got[:, :, rows:].any()
 
@@ -51,7 +51,7 @@ def test_split_gemm_pairs_pieces_exactly(ops):
     rng = np.random.default_rng(5)
     M, N, K = 200, 300, 640
     for P in (2, 3):
-        kp, npad, _ = ops.split_layout(P, M, N, K)
+        kp, mpad, npad, _ = ops.split_layout(P, M, N, K)
         Lv = rng.integers(-8, 9, (M, K)).astype(np.float32)
         Rv = rng.integers(-8, 9, (N, K)).astype(np.float32)
         Lb = (Lv.view(np.uint32) >> 16).astype(np.uint16)  # small integers are exact BF16
@@ -59,12 +59,12 @@ def test_split_gemm_pairs_pieces_exactly(ops):
         exact = Lv.astype(np.float64) @ Rv.astype(np.float64).T
         for p in range(P):
             for q in range(P):
-                Lp = np.zeros((P, M, kp), dtype=np.uint16)
+                Lp = np.zeros((P, mpad, kp), dtype=np.uint16)
                 Rp = np.zeros((P, npad, kp), dtype=np.uint16)
-                Lp[p, :, :K] = Lb
+                Lp[p, :M, :K] = Lb
                 Rp[q, :N, :K] = Rb
-                C = ops.split_gemm(ops.SplitPieces(torch.from_numpy(Lp).cuda(), M, K),
-                                   ops.SplitPieces(torch.from_numpy(Rp).cuda(), N, K)).cpu().numpy()
+                C = ops.split_gemm(ops.SplitPieces(ops.block_pieces(Lp).cuda(), M, K),
+                                   ops.SplitPieces(ops.block_pieces(Rp).cuda(), N, K)).cpu().numpy()
                 want = exact if p + q < P else np.zeros_like(exact)
                 assert np.array_equal(C, want), (P, p, q)
 
@@ -77,7 +77,7 @@ def test_split_gemm_matches_oracle(ops, P, M, N, K, tout):
     rng = np.random.default_rng(M + N + K + P)
     L = rng.standard_normal((M, K))
     R = rng.standard_normal((N, K))
-    kp, npad, _ = ops.split_layout(P, M, N, K)
+    kp, mpad, npad, _ = ops.split_layout(P, M, N, K)
     Lp = ops.split_bf16(torch.from_numpy(L).cuda(), P)
     Rp = ops.split_bf16(torch.from_numpy(R).cuda(), P, rows_out=npad)
     C = ops.split_gemm(Lp, Rp, transpose_out=tout).cpu().numpy()
