@@ -24,6 +24,47 @@ constexpr int POTRF_THREADS = 512;
 constexpr int SYRK_G = 4;  // trailing-update tiles per wavefront whose loads are issued together
 
 
+// Wavefront 0 factors the 32 x 32 diagonal block D = R11^T R11 held in Rd:
+// lane c holds column c in registers; at step j the finished row j of R11
+// goes to LDS (Rd row j, 1 / R_jj to rinv) and every lane reads it back with
+// same-address (broadcast) reads.  Returns 0, or j + 1 when pivot j is not
+// positive and finite (the wavefront stops there).
+__device__ __forceinline__ int potrf_factor32(double (*Rd)[CB + 1], double* rinv, int lane) {
+  const int c = pamd_opaque(lane & (CB - 1));
+  double col[CB];
+#pragma unroll
+  for (int i = 0; i < CB; ++i) col[i] = Rd[i][c];
+  PAMD_WAVE_SYNC();  // Rd is overwritten row by row below
+  int fail = 0;
+#pragma unroll
+  for (int j = 0; j < CB; ++j) {
+    const double djj = wave_bcast(col[j], j);
+    if (!fail && !(djj > 0.0 && djj < INFINITY)) fail = j + 1;
+    if (fail) continue;  // wave-uniform: djj is a broadcast
+    // 1/sqrt(djj): hardware estimate + two Newton steps y <- y (3 - d y^2) / 2
+    // (each squares the relative error), then sqrt(djj) = djj * y
+    double y = __builtin_amdgcn_rsq(djj);
+    const double h = 0.5 * djj;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    const double r = djj * y;
+    // R11[j][c] for c >= j; lanes c < j produce entries below the diagonal
+    // that no later step reads (lanes 32..63 repeat lanes 0..31)
+    const double v = (c == j) ? r : col[j] * y;
+    Rd[j][c] = v;
+    if (lane == 0) rinv[j] = y;
+    PAMD_WAVE_SYNC();
+    double rowj[CB];
+#pragma unroll
+    for (int i = 0; i < CB; ++i) rowj[i] = (i > j) ? Rd[j][i] : 0.0;
+    __builtin_amdgcn_sched_barrier(0);  // issue the row's reads together, then wait once
+#pragma unroll
+    for (int i = 0; i < CB; ++i)  // constant trip count: unrolls before the j loop does
+      if (i > j) col[i] -= rowj[i] * v;
+  }
+  return fail;
+}
+
 // Blocked right-looking upper Cholesky in one 512-thread workgroup, 32-wide
 // blocks.  Per block (G = the not yet factored part of A):
 //   1. wavefront 0 factors the 32 x 32 diagonal block D = R11^T R11: lane c
@@ -66,38 +107,7 @@ __device__ __forceinline__ void potrf_block(int64_t n, double* __restrict__ A, i
     }
     __syncthreads();
     if (wave == 0) {
-      const int c = pamd_opaque(lane & (CB - 1));
-      double col[CB];
-#pragma unroll
-      for (int i = 0; i < CB; ++i) col[i] = Rd[i][c];
-      PAMD_WAVE_SYNC();  // Rd is overwritten row by row below
-      int fail = 0;
-#pragma unroll
-      for (int j = 0; j < CB; ++j) {
-        const double djj = wave_bcast(col[j], j);
-        if (!fail && !(djj > 0.0 && djj < INFINITY)) fail = j + 1;
-        if (fail) continue;  // wave-uniform: djj is a broadcast
-        // 1/sqrt(djj): hardware estimate + two Newton steps y <- y (3 - d y^2) / 2
-        // (each squares the relative error), then sqrt(djj) = djj * y
-        double y = __builtin_amdgcn_rsq(djj);
-        const double h = 0.5 * djj;
-        y = y * fma(-h * y, y, 1.5);
-        y = y * fma(-h * y, y, 1.5);
-        const double r = djj * y;
-        // R11[j][c] for c >= j; lanes c < j produce entries below the diagonal
-        // that no later step reads (lanes 32..63 repeat lanes 0..31)
-        const double v = (c == j) ? r : col[j] * y;
-        Rd[j][c] = v;
-        if (lane == 0) rinv[j] = y;
-        PAMD_WAVE_SYNC();
-        double rowj[CB];
-#pragma unroll
-        for (int i = 0; i < CB; ++i) rowj[i] = (i > j) ? Rd[j][i] : 0.0;
-        __builtin_amdgcn_sched_barrier(0);  // issue the row's reads together, then wait once
-#pragma unroll
-        for (int i = 0; i < CB; ++i)  // constant trip count: unrolls before the j loop does
-          if (i > j) col[i] -= rowj[i] * v;
-      }
+      const int fail = potrf_factor32(Rd, rinv, lane);
       if (fail && lane == 0 && fail <= nb) bad = (int)(jb + fail);
     }
     __syncthreads();
@@ -109,25 +119,27 @@ __device__ __forceinline__ void potrf_block(int64_t n, double* __restrict__ A, i
     if (nt == 0) break;
     // 2. block row, thread per column (nt < POTRF_THREADS, a single pass)
     static_assert(POTRF_MAX_N - CB <= POTRF_THREADS, "one thread per trailing column");
-    if (tid < nt) {
+    if (tid < nt) {  // block row R12 = R11^{-T} G12, column c = tid: forward substitution in
+       // axpy order (row l, once known, updates all later rows), R11 broadcast
+       // from LDS, multiplications by 1 / R_ll
       const int c = tid;
       double t[CB];
-#pragma unroll
+    #pragma unroll
       for (int l = 0; l < CB; ++l) t[l] = Rs[l][c];
-#pragma unroll
+    #pragma unroll
       for (int l = 0; l < CB; ++l) {
         double rowl[CB];
-#pragma unroll
+    #pragma unroll
         for (int i = 0; i < CB; ++i) rowl[i] = (i > l) ? Rd[l][i] : 0.0;
         const double ril = rinv[l];
         __builtin_amdgcn_sched_barrier(0);
         const double y = t[l] * ril;
         t[l] = y;
-#pragma unroll
+    #pragma unroll
         for (int i = 0; i < CB; ++i)
           if (i > l) t[i] -= rowl[i] * y;
       }
-#pragma unroll
+    #pragma unroll
       for (int l = 0; l < CB; ++l) Rs[l][c] = t[l];  // rows >= nb stay zero
     }
     __syncthreads();
@@ -186,9 +198,233 @@ __device__ __forceinline__ void potrf_block(int64_t n, double* __restrict__ A, i
   if (tid == 0) info[0] = bad;
 }
 
+// Look-ahead variant for n <= LA_MAXN (the CholeskyQR Gram matrices of the
+// power iteration): the block row of the current block and of the next one
+// live in two LDS buffers, so the next block is formed straight into LDS and
+// its factorisation overlaps the rest of the current trailing update.  Per
+// block, with R12 of the current block in Rs[cur]:
+//   (a) all wavefronts: the next block's diagonal block and block row
+//       (trailing tile rows 0 and 1) = A - R12^T R12, into Rd and Rs[cur ^ 1];
+//   (b) wavefront 0 factors that diagonal block while wavefronts 1..7 update
+//       the rest of the trailing upper triangle in global memory;
+//   (c) all: the next block row R12' = R11'^{-T} (...), written to A with R11'.
+constexpr int LA_MAXN = 288;
+constexpr int LA_W = LA_MAXN - CB;  // staged trailing width
+
+__device__ __forceinline__ void potrf_la(int64_t n, double* __restrict__ A, int64_t lda, int* __restrict__ info) {
+  __shared__ double Rd[CB][CB + 1];
+  __shared__ double rinv[CB];
+  __shared__ double Rs[2][CB][LA_W];
+  __shared__ int bad;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int nthr = POTRF_THREADS, nwaves = POTRF_THREADS / 64;
+  const int lr = lane & 15, lk = lane >> 4;
+  if (tid == 0) bad = 0;
+  // block 0: stage, factor, block row
+  {
+    const int nb = (int)imin64(CB, n);
+    const int nt = (int)(n - nb);
+    const int ntp = (nt + 15) & ~15;
+    for (int e = tid; e < CB * CB; e += nthr) {
+      const int i = e / CB, j = e % CB;
+      Rd[i][j] = (i < nb && j < nb) ? A[i * lda + j] : ((i == j) ? 1.0 : 0.0);
+    }
+    for (int e = tid; e < CB * ntp; e += nthr) {
+      const int i = e / ntp, c = e % ntp;
+      Rs[0][i][c] = (i < nb && c < nt) ? A[i * lda + nb + c] : 0.0;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const int fail = potrf_factor32(Rd, rinv, lane);
+      if (fail && lane == 0 && fail <= nb) bad = fail;
+    }
+    __syncthreads();
+    if (bad) {
+      if (tid == 0) info[0] = bad;
+      return;
+    }
+    for (int e = tid; e < nb * nb; e += nthr) {
+      const int i = e / nb, j = e % nb;
+      if (j >= i) A[i * lda + j] = Rd[i][j];
+    }
+    if (tid < nt) {
+      double(*R0)[LA_W] = Rs[0];
+      {  // block row R12 = R11^{-T} G12, column c = tid: forward substitution in
+         // axpy order (row l, once known, updates all later rows), R11 broadcast
+         // from LDS, multiplications by 1 / R_ll
+        const int c = tid;
+        double t[CB];
+      #pragma unroll
+        for (int l = 0; l < CB; ++l) t[l] = R0[l][c];
+      #pragma unroll
+        for (int l = 0; l < CB; ++l) {
+          double rowl[CB];
+      #pragma unroll
+          for (int i = 0; i < CB; ++i) rowl[i] = (i > l) ? Rd[l][i] : 0.0;
+          const double ril = rinv[l];
+          __builtin_amdgcn_sched_barrier(0);
+          const double y = t[l] * ril;
+          t[l] = y;
+      #pragma unroll
+          for (int i = 0; i < CB; ++i)
+            if (i > l) t[i] -= rowl[i] * y;
+        }
+      #pragma unroll
+        for (int l = 0; l < CB; ++l) R0[l][c] = t[l];  // rows >= nb stay zero
+      }
+      for (int i = 0; i < nb; ++i) A[i * lda + nb + tid] = Rs[0][i][tid];
+    }
+    __syncthreads();
+  }
+  int cur = 0;
+  for (int64_t jb = 0; jb < n; jb += CB) {
+    const int nb = (int)imin64(CB, n - jb);
+    const int64_t c0 = jb + nb;  // first row / column of the trailing matrix
+    const int nt = (int)(n - c0);
+    if (nt <= 0) break;
+    const int T = ((nt + 15) & ~15) >> 4;
+    const int nb2 = nt < CB ? nt : CB;  // the next block
+    const int nt2 = nt - nb2;
+    const int ntp2 = (nt2 + 15) & ~15;
+    double(*Rn)[LA_W] = Rs[cur ^ 1];
+    const double(*Rc)[LA_W] = Rs[cur];
+    // (a) trailing tile rows 0 and 1 (the next block's rows) into LDS; padding
+    //     of the next diagonal block (identity) and block row (zeros) first
+    for (int e = tid; e < CB * CB; e += nthr) {
+      const int i = e / CB, j = e % CB;
+      if (i >= nb2 || j >= nb2) Rd[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+    for (int e = tid; e < CB * ntp2; e += nthr) {
+      const int i = e / ntp2, c = e % ntp2;
+      if (i >= nb2 || c >= nt2) Rn[i][c] = 0.0;
+    }
+    {
+      const int nta = (T >= 2) ? (2 * T - 1) : T;  // tiles (0, 0..T-1) and (1, 1..T-1)
+      for (int t = wave; t < nta; t += nwaves) {
+        const int ti = (t < T) ? 0 : 1, tc = (t < T) ? t : t - T + 1;
+        double av[CB / 4], bv[CB / 4], cv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = ti * 16 + lk + 4 * r, cc = tc * 16 + lr;
+          cv[r] = (i < nt && cc < nt && cc >= i) ? A[(c0 + i) * lda + c0 + cc] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < CB / 4; ++q) {
+          av[q] = Rc[4 * q + lk][ti * 16 + lr];
+          bv[q] = Rc[4 * q + lk][tc * 16 + lr];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < CB / 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = ti * 16 + lk + 4 * r, cc = tc * 16 + lr;
+          if (i < nb2 && cc < nt && cc >= i) {
+            const double v = cv[r] - acc[r];
+            if (cc < nb2)
+              Rd[i][cc] = v;
+            else
+              Rn[i][cc - nb2] = v;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // (b) factor the next diagonal block || the rest of the trailing update
+    if (wave == 0) {
+      const int fail = potrf_factor32(Rd, rinv, lane);
+      if (fail && lane == 0 && fail <= nb2) bad = (int)(c0 + fail);
+    } else if (T > 2) {
+      const int ntb = (T - 2) * (T - 1) / 2;  // tiles ti = 2 .. T-1, tc = ti .. T-1
+      for (int t0 = wave - 1; t0 < ntb; t0 += SYRK_G * (nwaves - 1)) {
+        int tis[SYRK_G], tcs[SYRK_G];
+        double cv[SYRK_G][4];
+#pragma unroll
+        for (int g = 0; g < SYRK_G; ++g) {
+          const int t = t0 + g * (nwaves - 1);
+          int ti = 2, rem = (t < ntb) ? t : 0;
+          while (rem >= T - ti) {
+            rem -= T - ti;
+            ++ti;
+          }
+          tis[g] = (t < ntb) ? ti : -1;
+          tcs[g] = ti + rem;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = ti * 16 + lk + 4 * r, cc = tcs[g] * 16 + lr;
+            cv[g][r] = (t < ntb && i < nt && cc < nt && cc >= i) ? A[(c0 + i) * lda + c0 + cc] : 0.0;
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < SYRK_G; ++g) {
+          if (tis[g] < 0) continue;  // wave-uniform
+          const int ti = tis[g], tc = tcs[g];
+          double av[CB / 4], bv[CB / 4];
+#pragma unroll
+          for (int q = 0; q < CB / 4; ++q) {
+            av[q] = Rc[4 * q + lk][ti * 16 + lr];
+            bv[q] = Rc[4 * q + lk][tc * 16 + lr];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int q = 0; q < CB / 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = ti * 16 + lk + 4 * r, cc = tc * 16 + lr;
+            if (i < nt && cc < nt && cc >= i) A[(c0 + i) * lda + c0 + cc] = cv[g][r] - acc[r];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (bad) break;
+    // (c) the next block's R11 and R12 to A
+    for (int e = tid; e < nb2 * nb2; e += nthr) {
+      const int i = e / nb2, j = e % nb2;
+      if (j >= i) A[(c0 + i) * lda + c0 + j] = Rd[i][j];
+    }
+    if (tid < nt2) {
+      {  // block row R12 = R11^{-T} G12, column c = tid: forward substitution in
+         // axpy order (row l, once known, updates all later rows), R11 broadcast
+         // from LDS, multiplications by 1 / R_ll
+        const int c = tid;
+        double t[CB];
+      #pragma unroll
+        for (int l = 0; l < CB; ++l) t[l] = Rn[l][c];
+      #pragma unroll
+        for (int l = 0; l < CB; ++l) {
+          double rowl[CB];
+      #pragma unroll
+          for (int i = 0; i < CB; ++i) rowl[i] = (i > l) ? Rd[l][i] : 0.0;
+          const double ril = rinv[l];
+          __builtin_amdgcn_sched_barrier(0);
+          const double y = t[l] * ril;
+          t[l] = y;
+      #pragma unroll
+          for (int i = 0; i < CB; ++i)
+            if (i > l) t[i] -= rowl[i] * y;
+        }
+      #pragma unroll
+        for (int l = 0; l < CB; ++l) Rn[l][c] = t[l];  // rows >= nb stay zero
+      }
+      for (int i = 0; i < nb2; ++i) A[(c0 + i) * lda + c0 + nb2 + tid] = Rn[i][tid];
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (tid == 0) info[0] = bad;
+}
+
 __global__ __launch_bounds__(POTRF_THREADS) void potrf_kernel(int64_t n, double* __restrict__ A, int64_t lda,
                                                               int* __restrict__ info) {
   potrf_block(n, A, lda, info);
+}
+
+__global__ __launch_bounds__(POTRF_THREADS) void potrf_la_kernel(int64_t n, double* __restrict__ A, int64_t lda,
+                                                                 int* __restrict__ info) {
+  potrf_la(n, A, lda, info);
 }
 
 // problem i: the n[i] x n[i] matrix at pool + off[i] (ld n[i]); one workgroup each
@@ -292,7 +528,10 @@ extern "C" int pamd_potrf_upper_f64(int64_t n, double* A, int64_t lda, int* info
   PAMD_ARG_CHECK(A != nullptr || n == 0, 2);
   PAMD_ARG_CHECK(lda >= n && lda >= 1, 3);
   PAMD_ARG_CHECK(info != nullptr, 4);
-  potrf_kernel<<<1, POTRF_THREADS, 0, (hipStream_t)stream>>>(n, A, lda, info);
+  if (n >= 1 && n <= LA_MAXN)
+    potrf_la_kernel<<<1, POTRF_THREADS, 0, (hipStream_t)stream>>>(n, A, lda, info);
+  else
+    potrf_kernel<<<1, POTRF_THREADS, 0, (hipStream_t)stream>>>(n, A, lda, info);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

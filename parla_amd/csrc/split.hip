@@ -258,22 +258,35 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
 }
 
 // C[i * lci + j * lcj] = sum over splits (in split order, FP64) of ws[ks][i][j];
-// 64 x 64 tiles through LDS so both the partial reads and the stores of
-// either orientation are coalesced
+// 64 x 64 tiles through LDS so both the partial reads (four columns per
+// thread) and the stores of either orientation are coalesced
 __global__ __launch_bounds__(256) void sg_reduce_kernel(int64_t M, int64_t N, int64_t npad, int nsplit,
                                                         const float* __restrict__ ws, double* __restrict__ C,
                                                         int64_t lci, int64_t lcj) {
   __shared__ double T[64][65];
   const int tid = threadIdx.x;
   const int64_t i0 = (int64_t)blockIdx.x * 64, j0 = (int64_t)blockIdx.y * 64;
-#pragma unroll 4
-  for (int q = 0; q < 16; ++q) {
-    const int ii = tid / 64 + 4 * q, jj = tid % 64;
-    const int64_t i = i0 + ii, j = j0 + jj;
-    double s = 0.0;
-    if (i < M && j < N)
-      for (int k = 0; k < nsplit; ++k) s += (double)ws[((int64_t)k * M + i) * npad + j];
-    T[ii][jj] = s;
+  const int jq = (tid & 15) * 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int ii = (tid >> 4) + 16 * q;
+    const int64_t i = i0 + ii, j = j0 + jq;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (i < M && j < npad) {
+      const float4* src = reinterpret_cast<const float4*>(ws + i * npad + j);
+      const int64_t stride = M * npad / 4;
+      for (int k = 0; k < nsplit; ++k) {
+        const float4 v = src[k * stride];
+        s0 += (double)v.x;
+        s1 += (double)v.y;
+        s2 += (double)v.z;
+        s3 += (double)v.w;
+      }
+    }
+    T[ii][jq] = s0;
+    T[ii][jq + 1] = s1;
+    T[ii][jq + 2] = s2;
+    T[ii][jq + 3] = s3;
   }
   __syncthreads();
   const bool by_row = (lcj == 1);  // contiguous along j: threads walk j
