@@ -567,8 +567,11 @@ def split_gemm(L, R, out=None, transpose_out=False):
     lci, lcj = (1, out.stride(0)) if transpose_out else (out.stride(0), 1)
     ws = workspace(wsb, dev)
     Lp, Rp = L.pieces, R.pieces
-    check(lib().pamd_split_gemm_f64(npieces, M, N, K, _p(Lp), Lp.shape[1] * kp, _p(Rp), Rp.shape[1] * kp,
-                                    _p(out), lci, lcj, _p(ws), stream(dev)), "pamd_split_gemm_f64")
+    npairs = npieces * (npieces + 1) // 2
+    # logged flops: the BF16 MFMA work executed (padded tile shapes, every piece pair)
+    with _Timed("split_gemm", 2.0 * mpad * npad * kp * npairs, 2.0 * npieces * (M + N) * K + 8.0 * M * N):
+        check(lib().pamd_split_gemm_f64(npieces, M, N, K, _p(Lp), Lp.shape[1] * kp, _p(Rp), Rp.shape[1] * kp,
+                                        _p(out), lci, lcj, _p(ws), stream(dev)), "pamd_split_gemm_f64")
     return out
 
 
