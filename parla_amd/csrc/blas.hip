@@ -188,8 +188,14 @@ __global__ void splitk_reduce_kernel(int64_t M, int64_t N, int splits, double al
 
 int gemm_splits(int64_t M, int64_t N, int64_t K) {
   const int64_t tiles = pamd_cdiv(M, GBM) * pamd_cdiv(N, GBN);
-  const int64_t target = 1024;  // workgroups to keep 256 CUs busy
-  if (tiles >= target / 2 || K < 4 * 256) return 1;
+  if (K < 4 * 256) return 1;
+  // a large output with a long K (the power iteration's A S, 640 tiles): split
+  // K until there are 2048 workgroups, eight per CU, so enough wavefronts are
+  // resident to hide MFMA and load latency; the few FP64 partials are cheap
+  // next to the K reduction
+  if (tiles >= 512) return (tiles >= 2048 || K < 2048) ? 1 : (int)pamd_cdiv(2048, tiles);
+  // small outputs: about 1024 workgroups, at least 256 K per slice
+  const int64_t target = 1024;
   int64_t s = pamd_cdiv(target, tiles);
   s = min(s, K / 256);  // keep >= 256 K per slice
   return (int)imax64(1, imin64(s, 64));
