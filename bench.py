@@ -123,11 +123,11 @@ def main():
     qb_bytes = 6.0 * m * n * 8
     value = qb_bytes / t_qb / 1e9
 
+    # errors on the device (FP64): no host BLAS between the timed legs
     Q, B = out["QB"]
-    Qh, Bh = Q.cpu().numpy(), B.cpu().numpy()
-    fro = np.linalg.norm(A_host)
-    qb_err = float(np.linalg.norm(A_host - Qh @ Bh) / fro)
-    orth_err = float(np.linalg.norm(Qh.T @ Qh - np.eye(Qh.shape[1])))
+    fro = float(torch.linalg.norm(A))
+    qb_err = float(torch.linalg.norm(A - Q @ B)) / fro
+    orth_err = float(torch.linalg.norm(Q.t() @ Q - torch.eye(Q.shape[1], dtype=Q.dtype, device=dev)))
     tail = float(np.sqrt(np.sum(s_true[k + over:] ** 2)) / np.sqrt(np.sum(s_true ** 2)))
 
     # roofline of the dominant kernel (the FP64 MFMA GEMM): per-launch events
@@ -152,7 +152,7 @@ def main():
 
         t_s = timed_steps(qbs_step, args.steps, args.warmup, torch)
         Qs_, Bs_ = out["QBS"]
-        err_s = float(np.linalg.norm(A_host - Qs_.cpu().numpy() @ Bs_.cpu().numpy()) / fro)
+        err_s = float(torch.linalg.norm(A - Qs_ @ Bs_)) / fro
         ev = []
         ops.set_event_log(ev)
         qbs_step(600)
