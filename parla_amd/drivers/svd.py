@@ -19,15 +19,15 @@ def small_svd(B):
     """Thin SVD of a short-wide (or small) device matrix B (kb x n).
 
     B^T = Q2 R2 by the library's Householder QR, so B = R2^T Q2^T; the kb x kb
-    SVD R2^T = Ur diag(s) Vr^T (rocSOLVER through torch.linalg) gives
-    U = Ur, Vh = (Q2 Vr)^T.  The large factor never leaves the library.
+    SVD R2^T = Ur diag(s) Vr^T (rocSOLVER's dgesdd) gives U = Ur,
+    Vh = (Q2 Vr)^T.  The large factor never leaves the device.
     """
     kb, n = B.shape
     if kb <= n:
         f = ops.geqrf_(ops.transpose(B))
         Q2 = f.Q()
         R2 = torch.triu(f.R_view)
-        Ur, s, Vrh = torch.linalg.svd(R2.t().contiguous(), full_matrices=False)
+        Ur, s, Vrh = ops.svd_small(R2.t())
         Vh = ops.transpose(ops.gemm(Q2, Vrh.t()))
         return Ur.contiguous(), s, Vh
     f = ops.geqrf_(B.contiguous().clone())
