@@ -261,42 +261,48 @@ int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, double* x_orig,
 /* Mixed precision: exact BF16 sketch, compensated residual, CG operator      */
 /* ------------------------------------------------------------------------ */
 
-/* Quantised Gaussian sketch Q (rows x m, stored as BF16 bit patterns, row-major,
- * ldq >= kp = m rounded up to a multiple of 256, ldq % 8 == 0): Q[i][k] =
- * rint(32 z_ik) clipped to [-256, 256] with z the same Philox stream as
- * pamd_gaussian_fill_f64 (counter offset + i*m + k); columns m..kp-1 are zero.
- * The sketching operator is S = Q * 2^-5 / sqrt(min(rows, m)), the
- * normalisation of utils/sketching.py:23 on an integer grid, so that the
- * BF16 sketch product below is exact.  Replaces rng.normal of
- * utils/sketching.py:24 for the mixed-precision sketch. */
-int pamd_gauss_q_bf16(int64_t rows, int64_t m, uint64_t seed, uint64_t offset,
-                      uint16_t* Q, int64_t ldq, void* stream);
+/* Shapes of the exact BF16 sketch of d rows over m rows of [A | b] (ncols
+ * columns): kp = m rounded up to 256 (K padding), dpad = d rounded up to 128
+ * (rows of Q's array), ncp = ncols rounded up to 64 (rows of each piece
+ * array), group_chunks = 256-row chunks per K group (the FP64 summation
+ * order), and the workspace of pamd_ozaki_sketch_f64. */
+int pamd_ozaki_layout(int64_t d, int64_t m, int64_t ncols, int64_t* kp, int64_t* dpad,
+                      int64_t* ncp, int64_t* group_chunks, int64_t* workspace_bytes);
+
+/* Quantised Gaussian sketch Q: Q[i][k] = rint(32 z_ik) clipped to [-256, 256]
+ * with z the same Philox stream as pamd_gaussian_fill_f64 (counter
+ * offset + i*m + k), stored as BF16 bit patterns in the blocked layout of
+ * pamd_split_bf16 (rows_out x kp, rows_out a multiple of 16, zeros for
+ * i >= rows or k >= m).  The sketching operator is S = Q * 2^-5 /
+ * sqrt(min(rows, m)), the normalisation of utils/sketching.py:23 on an
+ * integer grid, so that the BF16 sketch product below is exact.  Replaces
+ * rng.normal of utils/sketching.py:24 for the mixed-precision sketch. */
+int pamd_gauss_q_bf16(int64_t rows, int64_t rows_out, int64_t m, uint64_t seed,
+                      uint64_t offset, uint16_t* Q, void* stream);
 
 /* Error-free split of [A | b] (m x n, plus b as column n when b != NULL) into
  * npieces (1..5) BF16 integer pieces per 256-row chunk and column:
  * A[k][j] ~= 2^(e_cj - 8) sum_p 2^(-9p) a_p[k][j], |a_p| <= 256, with the
  * exponent e_cj (max |A[chunk c, j]| < 2^e_cj) in exps[c * lde + j].
- * pieces: npieces blocks of pstride elements, block p holding column j's
- * chunk-padded K run at pieces + p*pstride + j*ldk (ldk >= kp, ldk % 8 == 0,
- * pstride >= ncp*ldk, ncp = number of columns rounded up to 64).  First half
- * of the exact sketch of A (drivers/least_squares.py:303,314 in mixed
- * precision). */
+ * pieces: npieces blocked arrays (pstride >= ncp * kp elements apart) of
+ * ncp x kp, row j holding column j's K run.  First half of the exact sketch
+ * of A (drivers/least_squares.py:303,314 in mixed precision). */
 int pamd_ozaki_split_bf16(int64_t m, int64_t n, const double* A, int64_t lda,
                           const double* b, int npieces, uint16_t* pieces,
-                          int64_t ldk, int64_t pstride, int* exps, int64_t lde,
-                          void* stream);
+                          int64_t pstride, int* exps, int64_t lde, void* stream);
 
 /* out (d x ncols, FP64) = scale * Q @ [A | b] from the split above, on
  * v_mfma_f32_16x16x32_bf16: every 256-long partial sum of integer products is
- * exact in FP32 and is added in FP64 with its chunk scale.  The result equals
- * scale * Q @ (A truncated to 9*npieces bits below each chunk-column maximum)
- * up to the FP64 additions of the chunk sums.  Replaces S @ A and S @ b of
- * drivers/least_squares.py:303,314. */
+ * exact in FP32 and is added in FP64 with its chunk scale, chunk after chunk
+ * within a K group of group_chunks chunks; the group sums are then added in
+ * group order and scaled.  The result equals scale * Q @ (A truncated to
+ * 9*npieces bits below each chunk-column maximum) up to those FP64
+ * additions.  Q has dpad rows (pamd_ozaki_layout).  Replaces S @ A and S @ b
+ * of drivers/least_squares.py:303,314. */
 int pamd_ozaki_sketch_f64(int64_t d, int64_t m, int64_t ncols, int npieces,
-                          const uint16_t* Q, int64_t ldq, const uint16_t* pieces,
-                          int64_t ldk, int64_t pstride, const int* exps,
-                          int64_t lde, double scale, double* out, int64_t ldo,
-                          void* stream);
+                          const uint16_t* Q, const uint16_t* pieces, int64_t pstride,
+                          const int* exps, int64_t lde, double scale, double* out,
+                          int64_t ldo, void* workspace, void* stream);
 
 /* q = B^T (B p) in one read of B (m x n, n <= 1024, row-major; dtype 0: FP64,
  * 1: FP32 storage), FP64 vectors and accumulation, deterministic order.  The

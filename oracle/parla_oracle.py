@@ -617,19 +617,26 @@ def ozaki_split(A, b=None, npieces=4):
     return pieces, exps
 
 
-def ozaki_product(Q, pieces, exps, scale):
+def ozaki_product(Q, pieces, exps, scale, group_chunks=None):
     """scale * Q @ (truncated [A | b]) evaluated as the device does: exact
-    integer chunk sums, added in FP64 in the order chunk-major, piece-minor."""
+    integer chunk sums, added in FP64 in the order chunk-major, piece-minor
+    within each group of `group_chunks` consecutive chunks (all chunks when
+    None), then the group sums added in group order."""
     Qi = np.asarray(Q, dtype=np.int64)
     P, m, nc = pieces.shape
     d = Qi.shape[0]
-    acc = np.zeros((d, nc))
-    for c in range(exps.shape[0]):
-        sl = slice(c * OZ_CHUNK, min((c + 1) * OZ_CHUNK, m))
-        for p in range(P):
-            s = Qi[:, sl] @ pieces[p, sl].astype(np.int64)  # exact
-            acc += s.astype(np.float64) * np.ldexp(1.0, exps[c] - 8 - 9 * p)[None, :]
-    return scale * acc
+    nchunks = exps.shape[0]
+    gc = nchunks if group_chunks is None else int(group_chunks)
+    total = np.zeros((d, nc))
+    for c0 in range(0, nchunks, gc):
+        acc = np.zeros((d, nc))
+        for c in range(c0, min(c0 + gc, nchunks)):
+            sl = slice(c * OZ_CHUNK, min((c + 1) * OZ_CHUNK, m))
+            for p in range(P):
+                s = Qi[:, sl] @ pieces[p, sl].astype(np.int64)  # exact
+                acc += s.astype(np.float64) * np.ldexp(1.0, exps[c] - 8 - 9 * p)[None, :]
+        total += acc
+    return scale * total
 
 
 def _split26(v):

@@ -66,11 +66,11 @@ SIGNATURES = {
     "pamd_trsm_mfma_workspace": (c_int, [c_i64, ctypes.POINTER(c_i64)]),
     "pamd_trsm_right_upper_mfma_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
     "pamd_trsm_right_upper_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
-    "pamd_gauss_q_bf16": (c_int, [c_i64, c_i64, ctypes.c_uint64, ctypes.c_uint64, c_vp, c_i64, c_vp]),
-    "pamd_ozaki_split_bf16": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_int, c_vp, c_i64, c_i64, c_vp, c_i64,
-                                      c_vp]),
-    "pamd_ozaki_sketch_f64": (c_int, [c_i64, c_i64, c_i64, c_int, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64,
-                                      c_dbl, c_vp, c_i64, c_vp]),
+    "pamd_ozaki_layout": (c_int, [c_i64, c_i64, c_i64] + [ctypes.POINTER(c_i64)] * 5),
+    "pamd_gauss_q_bf16": (c_int, [c_i64, c_i64, c_i64, ctypes.c_uint64, ctypes.c_uint64, c_vp, c_vp]),
+    "pamd_ozaki_split_bf16": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_int, c_vp, c_i64, c_vp, c_i64, c_vp]),
+    "pamd_ozaki_sketch_f64": (c_int, [c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64,
+                                      c_vp, c_vp]),
     "pamd_normal_pass_workspace": (c_int, [c_i64, c_i64, ctypes.POINTER(c_i64)]),
     "pamd_normal_pass": (c_int, [c_int, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "pamd_residual_workspace": (c_int, [c_i64, c_i64, ctypes.POINTER(c_i64)]),

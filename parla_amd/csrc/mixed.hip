@@ -24,6 +24,7 @@
 // 4. Right triangular solve X R = A for large m, recursive: 32-column leaves
 //    by substitution (one thread per row), the off-diagonal blocks by the
 //    FP64 MFMA GEMM.
+#include "blocked.h"
 #include "common.h"
 #include "rng.h"
 
@@ -34,9 +35,10 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int OZ_CHUNK = 256;       // K rows per exact FP32 accumulation
 constexpr int OZ_MAX_PIECES = 5;
-constexpr int OZ_TM = 128, OZ_TN = 64, OZ_TK = 64;
+constexpr int OZ_TM = 128, OZ_TN = 64;
 constexpr int OZ_THREADS = 512;     // 8 waves, 4 (M) x 2 (N), 32 x 32 outputs each
-constexpr int OZ_LDS_PAD = 8;       // bf16 elements of padding per LDS row (16 B)
+constexpr int OZ_TARGET_WG = 4096;  // K groups until the grid has about this many workgroups
+constexpr int OZ_BATCH = 2;         // K steps of 32 per LDS buffer (two buffers)
 
 __device__ __forceinline__ uint16_t bf16_bits_of_int(double a) {
   // a is an integer with |a| <= 256: exact in BF16 (8 significant bits)
@@ -53,29 +55,33 @@ __device__ __forceinline__ uint16_t gauss_q_value(double z) {
   return bf16_bits_of_int(fmin(256.0, fmax(-256.0, rint(32.0 * z))));
 }
 
-// grid (ceil(kp / 512), rows): thread t of block x covers columns k = 2 (256 x + t)
-// and k + 1 of row blockIdx.y; both normals of one Philox pair serve the two
-// columns when the pair is aligned with them (one generator call per two
-// entries), two calls otherwise.
-__global__ __launch_bounds__(256) void gauss_q_kernel(int64_t m, int64_t kp, uint64_t seed, uint64_t offset,
-                                                      uint16_t* __restrict__ Q, int64_t ldq) {
-  const int64_t i = blockIdx.y;
-  const int64_t k = 2 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-  if (k >= kp) return;
-  uint16_t v0 = 0, v1 = 0;
-  const uint64_t e0 = offset + (uint64_t)(i * m + k);
-  if (k < m) {
-    double z0, z1;
-    normal_pair(seed, e0 >> 1, z0, z1);
-    v0 = gauss_q_value((e0 & 1) ? z1 : z0);
-    if (k + 1 < m) {
-      const uint64_t e1 = e0 + 1;
-      if ((e1 >> 1) != (e0 >> 1)) normal_pair(seed, e1 >> 1, z0, z1);
-      v1 = gauss_q_value((e1 & 1) ? z1 : z0);
+// Blocked Q (rows_out x kp, the layout of blocked.h; zeros for i >= rows or
+// k >= m): one wavefront per 16 x 32 block, lane l = row l / 4, eight
+// consecutive k; the Philox pair of two consecutive entries is drawn once.
+__global__ __launch_bounds__(256) void gauss_q_kernel(int64_t rows, int64_t m, int64_t nblk, int64_t kbn,
+                                                      uint64_t seed, uint64_t offset, uint16_t* __restrict__ Q) {
+  const int64_t blk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (blk >= nblk) return;
+  const int lane = threadIdx.x & 63, r = lane >> 2, c = lane & 3;
+  const int64_t i = (blk / kbn) * 16 + r;
+  const int64_t k0 = (blk % kbn) * 32 + 8 * c;
+  uint16_t v[8];
+  uint64_t pair = ~0ull;
+  double z0 = 0.0, z1 = 0.0;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    v[t] = 0;
+    const int64_t k = k0 + t;
+    if (i < rows && k < m) {
+      const uint64_t e = offset + (uint64_t)(i * m + k);
+      if ((e >> 1) != pair) {
+        pair = e >> 1;
+        normal_pair(seed, pair, z0, z1);
+      }
+      v[t] = gauss_q_value((e & 1) ? z1 : z0);
     }
   }
-  Q[i * ldq + k] = v0;
-  if (k + 1 < kp) Q[i * ldq + k + 1] = v1;
+  *reinterpret_cast<uint4*>(Q + blk * 512 + r * 32 + ((c ^ (r >> 2)) << 3)) = blk_pack(v);
 }
 
 // ---------------------------------------------------------------------------
@@ -115,7 +121,7 @@ __global__ __launch_bounds__(256) void oz_split_kernel(int64_t m, int64_t n, con
   const int e = sexp[jc];
   for (int p = 0; p < P; ++p) {
     // pieces p of rows k0 .. k0+31, column j, in 4 stores of 8 BF16
-    uint16_t* dst = pieces + p * pstride + j * ldk + k0;
+    uint16_t* dst = pieces + p * pstride;
     for (int q = 0; q < 4; ++q) {
       uint16_t h[8];
 #pragma unroll
@@ -133,55 +139,91 @@ __global__ __launch_bounds__(256) void oz_split_kernel(int64_t m, int64_t n, con
       w.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
       w.z = (uint32_t)h[4] | ((uint32_t)h[5] << 16);
       w.w = (uint32_t)h[6] | ((uint32_t)h[7] << 16);
-      *reinterpret_cast<uint4*>(dst + 8 * q) = w;
+      *reinterpret_cast<uint4*>(dst + blk_off(j, k0 + 8 * q, ldk)) = w;
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// exact sketch GEMM: out[i][j] = scale * sum_c sum_p 2^(e_cj - 8 - 9p) Q[i, chunk c] . a_p[chunk c, j]
-// Q: d x kp (row-major, ldq), pieces: [P][ncp][ldk] (piece-major, column j's K run contiguous)
+// exact sketch GEMM:
+//   part[g][i][j] = sum over the chunks c of group g (in order) of
+//                   sum_p 2^(e_cj - 8 - 9p) Q[i, chunk c] . a_p[chunk c, j]
+// and out = scale * sum_g part[g] (groups in order).  Q (rows padded to 128)
+// and the pieces (rows = columns of [A | b], padded to 64) are blocked
+// (blocked.h).  One 512-thread workgroup per 128 x 64 tile and K group; the
+// wavefronts of one XCD share their K group and column tile.
+//
+// The K steps of 32 stream through LDS in batches of OZ_BATCH steps by direct
+// global -> LDS loads, double-buffered: batch j + 1 is loaded while batch j
+// is multiplied.  Direct-to-LDS loads of one wavefront can complete out of
+// issue order, so a partial vmcnt does not say which of them are in: every
+// batch starts with vmcnt(0) and a barrier, and no other global load is
+// waited on inside a batch (the chunk exponents for the next batch are loaded
+// with the next batch's blocks and consumed after the next vmcnt(0)).
 // ---------------------------------------------------------------------------
 template <int P>
-__global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(int64_t d, int64_t ncols, int64_t kp,
-                                                             const uint16_t* __restrict__ Q, int64_t ldq,
-                                                             const uint16_t* __restrict__ pieces, int64_t ldk,
+__global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(int64_t kp, int64_t group_chunks, int mtiles,
+                                                             int ntiles, int ngroups,
+                                                             const uint16_t* __restrict__ Q,
+                                                             const uint16_t* __restrict__ pieces,
                                                              int64_t pstride, const int* __restrict__ exps,
-                                                             int64_t lde, double scale, double* __restrict__ out,
-                                                             int64_t ldo) {
-  constexpr int LDR = OZ_TK + OZ_LDS_PAD;
-  __shared__ __attribute__((aligned(16))) uint16_t Qs[2][OZ_TM][LDR];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][P][OZ_TN][LDR];
+                                                             int64_t lde, int64_t d, int64_t ncols,
+                                                             int64_t ncp, double* __restrict__ part) {
+  constexpr int LB = OZ_TM / 16, RB = OZ_TN / 16;  // 1 KiB blocks per K step: Q rows, piece rows
+  constexpr int NBLK = LB + P * RB;
+  constexpr int NW = OZ_THREADS / 64;
+  constexpr int B = OZ_BATCH;
+  constexpr int CS = OZ_CHUNK / 32;  // K steps per chunk
+  static_assert(CS % B == 0, "a batch lies within one chunk");
+  __shared__ uint4 SA[B][NBLK][64], SB[B][NBLK][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 3, wn = wave >> 2;
   const int lr = lane & 15, lk = lane >> 4;
-  const int64_t i0 = (int64_t)blockIdx.x * OZ_TM, j0 = (int64_t)blockIdx.y * OZ_TN;
+  const int frag = blk_frag(lane);
+  const uint32_t total = (uint32_t)mtiles * ntiles * ngroups;
+  const uint32_t t = xcd_tile(blockIdx.x, total);
+  const int g = (int)(t / ((uint32_t)mtiles * ntiles));
+  const uint32_t rem = t % ((uint32_t)mtiles * ntiles);
+  const int nt = (int)(rem / mtiles), mt = (int)(rem % mtiles);
+  const int64_t kbn = kp >> 5;
+  const int64_t nchunks = kp / OZ_CHUNK;
+  const int64_t c0 = (int64_t)g * group_chunks;
+  const int64_t c1 = imin64(nchunks, c0 + group_chunks);
+  const int nbatch = (int)((c1 - c0) * CS / B);
+  const int64_t kb0 = c0 * CS;
+  const uint4* Qg = reinterpret_cast<const uint4*>(Q);
+  const uint4* Pg = reinterpret_cast<const uint4*>(pieces);
+  const int64_t ps4 = pstride >> 3;
 
-  // global -> register staging: Q tile 128 x 64 = 1024 16-byte chunks (2 per
-  // thread), each piece tile 64 x 64 = 512 chunks (1 per thread)
-  uint4 rq[2], rb[P];
-  auto load_tile = [&](int64_t k0) {
+  // wavefront w loads blocks w, w + NW, ... of every step of batch j
+  auto issue = [&](int j, uint4 (*dst)[NBLK][64]) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int ch = tid + OZ_THREADS * h;
-      const int row = ch >> 3, kc = (ch & 7) * 8;
-      const int64_t gi = i0 + row;
-      rq[h] = (gi < d) ? *reinterpret_cast<const uint4*>(Q + gi * ldq + k0 + kc) : uint4{0, 0, 0, 0};
-    }
+    for (int u = 0; u < B; ++u) {
+      const int64_t kb = kb0 + (int64_t)j * B + u;
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-      const int col = tid >> 3, kc = (tid & 7) * 8;
-      rb[p] = *reinterpret_cast<const uint4*>(pieces + p * pstride + (j0 + col) * ldk + k0 + kc);
+      for (int i = 0; i < (NBLK + NW - 1) / NW; ++i) {
+        const int b = wave + NW * i;
+        if (b < NBLK) {
+          const uint4* src;
+          if (b < LB) {
+            src = Qg + (((int64_t)mt * LB + b) * kbn + kb) * 64;
+          } else {
+            const int p = (b - LB) / RB, rb = (b - LB) % RB;
+            src = Pg + p * ps4 + (((int64_t)nt * RB + rb) * kbn + kb) * 64;
+          }
+          blk_load_lds(src, &dst[u][b][0], lane);
+        }
+      }
     }
   };
-  auto store_tile = [&](int buf) {
+  // exponents (this lane's two columns) of the chunk holding batch j
+  auto load_exps = [&](int j, int (&e)[2]) {
+    const int64_t cidx = c0 + ((int64_t)j * B) / CS;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int ch = tid + OZ_THREADS * h;
-      *reinterpret_cast<uint4*>(&Qs[buf][ch >> 3][(ch & 7) * 8]) = rq[h];
+    for (int nb = 0; nb < 2; ++nb) {
+      const int64_t col = (int64_t)nt * OZ_TN + 32 * wn + 16 * nb + lr;
+      e[nb] = (col < ncols) ? exps[cidx * lde + col] : 0;
     }
-#pragma unroll
-    for (int p = 0; p < P; ++p) *reinterpret_cast<uint4*>(&Bs[buf][p][tid >> 3][(tid & 7) * 8]) = rb[p];
   };
 
   f4 acc[P][2][2];
@@ -198,79 +240,109 @@ __global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(int64_t d, int64_t 
     for (int bq = 0; bq < 2; ++bq)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc64[a][bq][r] = 0.0;
+  int ecur[2] = {0, 0}, enext[2] = {0, 0};
 
-  const int64_t ntiles = kp / OZ_TK;
-  constexpr int TPC = OZ_CHUNK / OZ_TK;  // K tiles per chunk
-  load_tile(0);
-  store_tile(0);
-  __syncthreads();
-  if (ntiles > 1) load_tile(OZ_TK);
-  int ecol[2] = {0, 0};
-  for (int64_t t = 0; t < ntiles; ++t) {
-    const int buf = (int)(t & 1);
-    if (t % TPC == 0) {  // exponents of this chunk, used at its flush
-      const int64_t cidx = t / TPC;
-#pragma unroll
-      for (int nt = 0; nt < 2; ++nt) {
-        const int64_t j = j0 + 32 * wn + 16 * nt + lr;
-        ecol[nt] = (j < ncols) ? exps[cidx * lde + j] : 0;
-      }
+  auto batch = [&](int j, uint4 (*cur)[NBLK][64], uint4 (*nxt)[NBLK][64]) {
+    __builtin_amdgcn_s_waitcnt(0x3F70);  // vmcnt(0): this batch's blocks and exponents are in
+    asm volatile("s_barrier" ::: "memory");  // ... for every wavefront; the other buffer is free
+    ecur[0] = enext[0];
+    ecur[1] = enext[1];
+    if (j + 1 < nbatch) {
+      issue(j + 1, nxt);
+      load_exps(j + 1, enext);
     }
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
+    for (int u = 0; u < B; ++u) {
       bf16x8 a[2], bv[P][2];
 #pragma unroll
-      for (int mt = 0; mt < 2; ++mt)
-        a[mt] = *reinterpret_cast<const bf16x8*>(&Qs[buf][32 * wm + 16 * mt + lr][32 * ks + 8 * lk]);
+      for (int mb = 0; mb < 2; ++mb) a[mb] = *reinterpret_cast<const bf16x8*>(&cur[u][wm * 2 + mb][frag]);
 #pragma unroll
       for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt)
-          bv[p][nt] = *reinterpret_cast<const bf16x8*>(&Bs[buf][p][32 * wn + 16 * nt + lr][32 * ks + 8 * lk]);
+        for (int nb = 0; nb < 2; ++nb)
+          bv[p][nb] = *reinterpret_cast<const bf16x8*>(&cur[u][LB + p * RB + wn * 2 + nb][frag]);
 #pragma unroll
       for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
+        for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-          for (int nt = 0; nt < 2; ++nt)
-            acc[p][mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], bv[p][nt], acc[p][mt][nt], 0, 0, 0);
+          for (int nb = 0; nb < 2; ++nb)
+            acc[p][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], bv[p][nb], acc[p][mb][nb], 0, 0, 0);
     }
-    if (t + 1 < ntiles) {
-      store_tile(buf ^ 1);
-      if (t + 2 < ntiles) load_tile((t + 2) * OZ_TK);
-    }
-    if ((t + 1) % TPC == 0) {  // end of a chunk: exact FP32 sums -> FP64
+    if (((j + 1) * B) % CS == 0) {  // end of a chunk: exact FP32 sums -> FP64, pieces in order
 #pragma unroll
-      for (int nt = 0; nt < 2; ++nt) {
+      for (int nb = 0; nb < 2; ++nb) {
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-          const double sc = ldexp(1.0, ecol[nt] - 8 - 9 * p);
+          const double sc = ldexp(1.0, ecur[nb] - 8 - 9 * p);
 #pragma unroll
-          for (int mt = 0; mt < 2; ++mt)
+          for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc64[mt][nt][r] += (double)acc[p][mt][nt][r] * sc;
+            for (int r = 0; r < 4; ++r) acc64[mb][nb][r] += (double)acc[p][mb][nb][r] * sc;
         }
       }
 #pragma unroll
       for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
+        for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-          for (int nt = 0; nt < 2; ++nt) acc[p][mt][nt] = f4{0.f, 0.f, 0.f, 0.f};
+          for (int nb = 0; nb < 2; ++nb) acc[p][mb][nb] = f4{0.f, 0.f, 0.f, 0.f};
     }
-    __syncthreads();
+  };
+
+  if (nbatch > 0) {
+    issue(0, SA);
+    load_exps(0, enext);
+  }
+  for (int j = 0; j < nbatch; j += 2) {
+    batch(j, SA, SB);
+    if (j + 1 < nbatch) batch(j + 1, SB, SA);
   }
   // D layout of v_mfma_f32_16x16x32_bf16: lane l, item r -> row 4 (l / 16) + r, col l % 16
+  double* w = part + (int64_t)g * d * ncp;
 #pragma unroll
-  for (int mt = 0; mt < 2; ++mt)
+  for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt)
+    for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t i = i0 + 32 * wm + 16 * mt + 4 * lk + r;
-        const int64_t j = j0 + 32 * wn + 16 * nt + lr;
-        if (i < d && j < ncols) out[i * ldo + j] = scale * acc64[mt][nt][r];
+        const int64_t i = (int64_t)mt * OZ_TM + 32 * wm + 16 * mb + 4 * lk + r;
+        const int64_t j = (int64_t)nt * OZ_TN + 32 * wn + 16 * nb + lr;
+        if (i < d) w[i * ncp + j] = acc64[mb][nb][r];
       }
+}
+
+// out[i][j] = scale * sum_g part[g][i][j], groups in order
+__global__ __launch_bounds__(256) void oz_reduce_kernel(int64_t d, int64_t ncols, int64_t ncp, int ngroups,
+                                                        const double* __restrict__ part, double scale,
+                                                        double* __restrict__ out, int64_t ldo) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= d * ncols) return;
+  const int64_t i = idx / ncols, j = idx % ncols;
+  double s = 0.0;
+  for (int g = 0; g < ngroups; ++g) s += part[((int64_t)g * d + i) * ncp + j];
+  out[i * ldo + j] = scale * s;
+}
+
+struct OzShape {
+  int mtiles, ntiles, ngroups;
+  int64_t kp, ncp, mpad, group_chunks;
+};
+
+// K groups: about OZ_TARGET_WG workgroups in all, at least two chunks per
+// group (the group size is part of the summation order)
+OzShape oz_shape(int64_t d, int64_t m, int64_t ncols) {
+  OzShape s;
+  s.kp = pamd_cdiv(imax64(m, 1), OZ_CHUNK) * OZ_CHUNK;
+  s.ncp = pamd_cdiv(imax64(ncols, 1), OZ_TN) * OZ_TN;
+  s.mpad = pamd_cdiv(imax64(d, 1), OZ_TM) * OZ_TM;
+  s.mtiles = (int)(s.mpad / OZ_TM);
+  s.ntiles = (int)(s.ncp / OZ_TN);
+  const int64_t nchunks = s.kp / OZ_CHUNK;
+  const int64_t want = imax64(1, pamd_cdiv(OZ_TARGET_WG, (int64_t)s.mtiles * s.ntiles));
+  s.group_chunks = imax64(2, pamd_cdiv(nchunks, want));
+  s.ngroups = (int)pamd_cdiv(nchunks, s.group_chunks);
+  return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -602,75 +674,93 @@ inline int64_t np_grid(int64_t m) { return imax64(1, imin64(NP_MAX_GRID, pamd_cd
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-extern "C" int pamd_gauss_q_bf16(int64_t rows, int64_t m, uint64_t seed, uint64_t offset, uint16_t* Q, int64_t ldq,
-                                 void* stream) {
-  PAMD_ARG_CHECK(rows >= 0 && rows <= 65535, 1);
+extern "C" int pamd_ozaki_layout(int64_t d, int64_t m, int64_t ncols, int64_t* kp, int64_t* dpad, int64_t* ncp,
+                                 int64_t* group_chunks, int64_t* workspace_bytes) {
+  PAMD_ARG_CHECK(d >= 0, 1);
   PAMD_ARG_CHECK(m >= 0, 2);
-  const int64_t kp = pamd_cdiv(m, OZ_CHUNK) * OZ_CHUNK;
-  PAMD_ARG_CHECK(Q != nullptr || rows * kp == 0, 5);
-  PAMD_ARG_CHECK(ldq >= kp && ldq % 8 == 0, 6);
-  if (rows * kp == 0) return PAMD_OK;
-  dim3 grid((unsigned)pamd_cdiv(kp, 512), (unsigned)rows);
-  gauss_q_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(m, kp, seed, offset, Q, ldq);
+  PAMD_ARG_CHECK(ncols >= 0, 3);
+  PAMD_ARG_CHECK(kp && dpad && ncp && group_chunks && workspace_bytes, 4);
+  const OzShape s = oz_shape(d, m, ncols);
+  *kp = s.kp;
+  *dpad = s.mpad;
+  *ncp = s.ncp;
+  *group_chunks = s.group_chunks;
+  *workspace_bytes = (int64_t)s.ngroups * imax64(d, 1) * s.ncp * (int64_t)sizeof(double);
+  return PAMD_OK;
+}
+
+extern "C" int pamd_gauss_q_bf16(int64_t rows, int64_t rows_out, int64_t m, uint64_t seed, uint64_t offset,
+                                 uint16_t* Q, void* stream) {
+  PAMD_ARG_CHECK(rows >= 0, 1);
+  PAMD_ARG_CHECK(rows_out >= rows && rows_out % 16 == 0, 2);
+  PAMD_ARG_CHECK(m >= 0, 3);
+  const int64_t kp = pamd_cdiv(imax64(m, 1), OZ_CHUNK) * OZ_CHUNK;
+  PAMD_ARG_CHECK(Q != nullptr || rows_out == 0, 6);
+  if (rows_out == 0) return PAMD_OK;
+  const int64_t kbn = kp / 32, nblk = (rows_out / 16) * kbn;
+  PAMD_ARG_CHECK(pamd_cdiv(nblk, 4) < (1LL << 31), 2);
+  gauss_q_kernel<<<(unsigned)pamd_cdiv(nblk, 4), 256, 0, (hipStream_t)stream>>>(rows, m, nblk, kbn, seed, offset, Q);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }
 
 extern "C" int pamd_ozaki_split_bf16(int64_t m, int64_t n, const double* A, int64_t lda, const double* b,
-                                     int npieces, uint16_t* pieces, int64_t ldk, int64_t pstride, int* exps,
-                                     int64_t lde, void* stream) {
+                                     int npieces, uint16_t* pieces, int64_t pstride, int* exps, int64_t lde,
+                                     void* stream) {
   const int64_t ncols = n + (b != nullptr ? 1 : 0);
-  const int64_t kp = pamd_cdiv(m, OZ_CHUNK) * OZ_CHUNK;
-  const int64_t ncp = pamd_cdiv(ncols, OZ_TN) * OZ_TN;
+  const int64_t kp = pamd_cdiv(imax64(m, 1), OZ_CHUNK) * OZ_CHUNK;
+  const int64_t ncp = pamd_cdiv(imax64(ncols, 1), OZ_TN) * OZ_TN;
   PAMD_ARG_CHECK(m >= 0, 1);
   PAMD_ARG_CHECK(n >= 0, 2);
   PAMD_ARG_CHECK(A != nullptr || m * n == 0, 3);
   PAMD_ARG_CHECK(lda >= n && lda >= 1, 4);
   PAMD_ARG_CHECK(npieces >= 1 && npieces <= OZ_MAX_PIECES, 6);
-  PAMD_ARG_CHECK(pieces != nullptr || kp * ncp == 0, 7);
-  PAMD_ARG_CHECK(ldk >= kp && ldk % 8 == 0, 8);
-  PAMD_ARG_CHECK(pstride >= ncp * ldk, 9);
-  PAMD_ARG_CHECK(exps != nullptr || kp == 0, 10);
-  PAMD_ARG_CHECK(lde >= ncols, 11);
-  if (kp == 0 || ncp == 0) return PAMD_OK;
+  PAMD_ARG_CHECK(pieces != nullptr, 7);
+  PAMD_ARG_CHECK(pstride >= ncp * kp && pstride % 8 == 0, 8);
+  PAMD_ARG_CHECK(exps != nullptr, 9);
+  PAMD_ARG_CHECK(lde >= ncols, 10);
   dim3 grid((unsigned)(kp / OZ_CHUNK), (unsigned)(ncp / 32));
-  oz_split_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(m, n, A, lda, b, npieces, pieces, ldk, pstride, exps, lde);
+  oz_split_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(m, n, A, lda, b, npieces, pieces, kp, pstride, exps, lde);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }
 
 extern "C" int pamd_ozaki_sketch_f64(int64_t d, int64_t m, int64_t ncols, int npieces, const uint16_t* Q,
-                                     int64_t ldq, const uint16_t* pieces, int64_t ldk, int64_t pstride,
-                                     const int* exps, int64_t lde, double scale, double* out, int64_t ldo,
-                                     void* stream) {
-  const int64_t kp = pamd_cdiv(m, OZ_CHUNK) * OZ_CHUNK;
-  const int64_t ncp = pamd_cdiv(ncols, OZ_TN) * OZ_TN;
+                                     const uint16_t* pieces, int64_t pstride, const int* exps, int64_t lde,
+                                     double scale, double* out, int64_t ldo, void* workspace, void* stream) {
   PAMD_ARG_CHECK(d >= 0, 1);
-  PAMD_ARG_CHECK(m >= 0, 2);
+  PAMD_ARG_CHECK(m >= 1, 2);
   PAMD_ARG_CHECK(ncols >= 0, 3);
   PAMD_ARG_CHECK(npieces >= 1 && npieces <= OZ_MAX_PIECES, 4);
-  PAMD_ARG_CHECK(Q != nullptr || d * kp == 0, 5);
-  PAMD_ARG_CHECK(ldq >= kp && ldq % 8 == 0, 6);
-  PAMD_ARG_CHECK(pieces != nullptr || kp * ncp == 0, 7);
-  PAMD_ARG_CHECK(ldk >= kp && ldk % 8 == 0, 8);
-  PAMD_ARG_CHECK(pstride >= ncp * ldk, 9);
-  PAMD_ARG_CHECK(exps != nullptr || kp == 0, 10);
-  PAMD_ARG_CHECK(lde >= ncols, 11);
-  PAMD_ARG_CHECK(out != nullptr || d * ncols == 0, 13);
-  PAMD_ARG_CHECK(ldo >= ncols && ldo >= 1, 14);
   if (d == 0 || ncols == 0) return PAMD_OK;
-  if (kp == 0) {
-    return PAMD_ERR_UNSUPPORTED;  // m == 0: nothing to sketch; the caller zeroes `out`
-  }
-  dim3 grid((unsigned)pamd_cdiv(d, OZ_TM), (unsigned)(ncp / OZ_TN));
+  const OzShape s = oz_shape(d, m, ncols);
+  PAMD_ARG_CHECK(Q != nullptr, 5);
+  PAMD_ARG_CHECK(pieces != nullptr, 6);
+  PAMD_ARG_CHECK(pstride >= s.ncp * s.kp && pstride % 8 == 0, 7);
+  PAMD_ARG_CHECK(exps != nullptr, 8);
+  PAMD_ARG_CHECK(lde >= ncols, 9);
+  PAMD_ARG_CHECK(out != nullptr, 11);
+  PAMD_ARG_CHECK(ldo >= ncols && ldo >= 1, 12);
+  PAMD_ARG_CHECK(workspace != nullptr, 13);
+  const int64_t total = (int64_t)s.mtiles * s.ntiles * s.ngroups;
+  PAMD_ARG_CHECK(total < (1LL << 31), 1);
   hipStream_t st = (hipStream_t)stream;
+  double* part = (double*)workspace;
+#define PAMD_OZ(P)                                                                                       \
+  oz_gemm_kernel<P><<<(unsigned)total, OZ_THREADS, 0, st>>>(s.kp, s.group_chunks, s.mtiles, s.ntiles,    \
+                                                            s.ngroups, Q, pieces, pstride, exps, lde, d,  \
+                                                            ncols, s.ncp, part)
   switch (npieces) {
-    case 1: oz_gemm_kernel<1><<<grid, OZ_THREADS, 0, st>>>(d, ncols, kp, Q, ldq, pieces, ldk, pstride, exps, lde, scale, out, ldo); break;
-    case 2: oz_gemm_kernel<2><<<grid, OZ_THREADS, 0, st>>>(d, ncols, kp, Q, ldq, pieces, ldk, pstride, exps, lde, scale, out, ldo); break;
-    case 3: oz_gemm_kernel<3><<<grid, OZ_THREADS, 0, st>>>(d, ncols, kp, Q, ldq, pieces, ldk, pstride, exps, lde, scale, out, ldo); break;
-    case 4: oz_gemm_kernel<4><<<grid, OZ_THREADS, 0, st>>>(d, ncols, kp, Q, ldq, pieces, ldk, pstride, exps, lde, scale, out, ldo); break;
-    default: oz_gemm_kernel<5><<<grid, OZ_THREADS, 0, st>>>(d, ncols, kp, Q, ldq, pieces, ldk, pstride, exps, lde, scale, out, ldo); break;
+    case 1: PAMD_OZ(1); break;
+    case 2: PAMD_OZ(2); break;
+    case 3: PAMD_OZ(3); break;
+    case 4: PAMD_OZ(4); break;
+    default: PAMD_OZ(5); break;
   }
+#undef PAMD_OZ
+  PAMD_LAUNCH_CHECK();
+  oz_reduce_kernel<<<(unsigned)pamd_cdiv(d * ncols, 256), 256, 0, st>>>(d, ncols, s.ncp, s.ngroups, part, scale,
+                                                                       out, ldo);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

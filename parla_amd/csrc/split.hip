@@ -29,7 +29,7 @@
 // assigned XCD-aware: the workgroups of one XCD share their K range and
 // column tile, so the right operand's slice is fetched into that XCD's L2
 // once.
-#include "common.h"
+#include "blocked.h"
 
 typedef __bf16 sg_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float sg_f4 __attribute__((ext_vector_type(4)));
@@ -60,21 +60,6 @@ __device__ __forceinline__ void sg_split(double x, uint16_t (&h)[P]) {
   }
 }
 
-__device__ __forceinline__ uint4 sg_pack(const uint16_t (&v)[8]) {
-  uint4 w;
-  w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
-  w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
-  w.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
-  w.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
-  return w;
-}
-
-// offset (in BF16 elements) of element (i, k) in a blocked piece array
-__device__ __forceinline__ int64_t sg_off(int64_t i, int64_t k, int64_t kp) {
-  const int r = (int)(i & 15), c = (int)((k & 31) >> 3);
-  return ((i >> 4) * (kp >> 5) + (k >> 5)) * 512 + r * 32 + ((c ^ (r >> 2)) << 3) + (k & 7);
-}
-
 // blocked pieces of X (rows x cols, row-major): one wavefront per 16 x 32
 // block, lane l = row l / 4, chunk l % 4 (eight consecutive k)
 template <int P>
@@ -98,7 +83,7 @@ __global__ __launch_bounds__(256) void sg_split_rows_kernel(int64_t rows, int64_
   }
   const int64_t off = blk * 512 + r * 32 + ((c ^ (r >> 2)) << 3);
 #pragma unroll
-  for (int p = 0; p < P; ++p) *reinterpret_cast<uint4*>(pieces + p * pstride + off) = sg_pack(h[p]);
+  for (int p = 0; p < P; ++p) *reinterpret_cast<uint4*>(pieces + p * pstride + off) = blk_pack(h[p]);
 }
 
 // blocked pieces of X^T (row j of the piece array = column j of X, K = X's
@@ -133,21 +118,21 @@ __global__ __launch_bounds__(256) void sg_split_t_kernel(int64_t rows, int64_t c
 #pragma unroll
       for (int p = 0; p < P; ++p) v[p][e] = t[p];
     }
-    const int64_t off = sg_off(j, i0 + kr * 32 + 8 * c, kp);
+    const int64_t off = blk_off(j, i0 + kr * 32 + 8 * c, kp);
 #pragma unroll
-    for (int p = 0; p < P; ++p) *reinterpret_cast<uint4*>(pieces + p * pstride + off) = sg_pack(v[p]);
+    for (int p = 0; p < P; ++p) *reinterpret_cast<uint4*>(pieces + p * pstride + off) = blk_pack(v[p]);
   }
 }
 
-typedef __attribute__((address_space(3))) void sg_lds_void;
-typedef const __attribute__((address_space(1))) void sg_glb_void;
-
 // ws[ks][i][j] (FP32) = sum over p + q < P of L_p[i, K_ks] R_q[j, K_ks]^T.
-// K steps stream through three LDS stages by direct global -> LDS loads
+// K steps stream through two LDS buffers by direct global -> LDS loads
 // (global_load_lds_dwordx4: a wavefront moves one 1 KiB block per
-// instruction, no registers involved); the loads for step s + 2 are issued
-// during step s.  The stages are separate arrays and the loop is unrolled by
-// three, so every access names its stage at compile time.
+// instruction, no registers involved): step s + 1 is loaded while step s is
+// multiplied.  Direct-to-LDS loads of one wavefront can complete out of issue
+// order, so a partial vmcnt does not say which of them are in: every step
+// starts with vmcnt(0) and a barrier.  The buffers are separate arrays and
+// the loop is unrolled by two, so every access names its buffer at compile
+// time.
 template <int P, int WM, int WN, int MB, int NB>
 __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t kc, int mtiles, int ntiles,
                                                              int nsplit, const uint16_t* __restrict__ L,
@@ -158,12 +143,11 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
   static_assert(TM == SG_TM && WM * WN == 4, "four wavefronts on a 128-row tile");
   constexpr int LB = TM / 16, RB = TN / 16;  // 1 KiB blocks per piece per K step
   constexpr int NBLK = P * (LB + RB);        // blocks per K step: L pieces, then R pieces
-  constexpr int PERW = NBLK / 4;             // blocks every wavefront loads per step (at least)
-  __shared__ uint4 S0[NBLK][64], S1[NBLK][64], S2[NBLK][64];
+  __shared__ uint4 S0[NBLK][64], S1[NBLK][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
+  const int frag = blk_frag(lane);
   const int lr = lane & 15, lk = lane >> 4;
-  const int frag = lr * 4 + (lk ^ (lr >> 2));  // this lane's swizzled 16-byte chunk in a block
   const uint32_t total = (uint32_t)mtiles * ntiles * nsplit;
   const uint32_t t = xcd_tile(blockIdx.x, total);
   const int ks = (int)(t / ((uint32_t)mtiles * ntiles));
@@ -172,11 +156,11 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
   const int64_t kbn = kp >> 5;  // K blocks per row block
   const int64_t kb0 = ((int64_t)ks * kc) >> 5;
   const int nsteps = (int)((imin64(kp, (int64_t)(ks + 1) * kc) - (int64_t)ks * kc) >> 5);
-  const uint4* Lg = reinterpret_cast<const uint4*>(L) + lane;
-  const uint4* Rg = reinterpret_cast<const uint4*>(R) + lane;
+  const uint4* Lg = reinterpret_cast<const uint4*>(L);
+  const uint4* Rg = reinterpret_cast<const uint4*>(R);
   const int64_t lps4 = lps >> 3, rps4 = rps >> 3;
 
-  // wavefront w loads blocks w, w + 4, ... of step `step` into stage `dst`
+  // wavefront w loads blocks w, w + 4, ... of step `step` into buffer `dst`
   auto issue = [&](int step, uint4 (*dst)[64]) {
     const int64_t kb = kb0 + step;
 #pragma unroll
@@ -191,7 +175,7 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
           const int q = (b - P * LB) / RB, rb = (b - P * LB) % RB;
           src = Rg + q * rps4 + (((int64_t)nt * RB + rb) * kbn + kb) * 64;
         }
-        __builtin_amdgcn_global_load_lds((sg_glb_void*)src, (sg_lds_void*)&dst[b][0], 16, 0, 0);
+        blk_load_lds(src, &dst[b][0], lane);
       }
     }
   };
@@ -202,13 +186,14 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = sg_f4{0.f, 0.f, 0.f, 0.f};
 
-  // one K step: wait for this wavefront's loads of the step (the loads of
-  // the next step may stay in flight), barrier (every wavefront's loads are
-  // in and the stage read two steps ago is free), read the fragments, issue
-  // the loads two steps ahead, then the MFMAs
-  auto step = [&](int s, uint4 (*cur)[64], uint4 (*ahead)[64]) {
-    __builtin_amdgcn_s_waitcnt((PERW & 15) | ((PERW >> 4) << 14) | 0x3F70);  // vmcnt(PERW)
+  // one K step: wait for every load of this wavefront (the step's blocks),
+  // barrier (every wavefront's blocks are in, and the other buffer, read in
+  // the previous step, is free), issue the next step's loads into it, then
+  // read the fragments and run the MFMAs
+  auto step = [&](int s, uint4 (*cur)[64], uint4 (*nxt)[64]) {
+    __builtin_amdgcn_s_waitcnt(0x3F70);  // vmcnt(0)
     asm volatile("s_barrier" ::: "memory");
+    if (s + 1 < nsteps) issue(s + 1, nxt);
     sg_bf16x8 a[P][MB], b[P][NB];
 #pragma unroll
     for (int p = 0; p < P; ++p)
@@ -220,8 +205,6 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb)
         b[q][nb] = *reinterpret_cast<const sg_bf16x8*>(&cur[P * LB + q * RB + wn * NB + nb][frag]);
-    asm volatile("" ::: "memory");
-    if (s + 2 < nsteps) issue(s + 2, ahead);
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
@@ -236,11 +219,9 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
   };
 
   if (nsteps > 0) issue(0, S0);
-  if (nsteps > 1) issue(1, S1);
-  for (int s = 0; s < nsteps; s += 3) {
-    step(s, S0, S2);
+  for (int s = 0; s < nsteps; s += 2) {
+    step(s, S0, S1);
     if (s + 1 < nsteps) step(s + 1, S1, S0);
-    if (s + 2 < nsteps) step(s + 2, S2, S1);
   }
   // D layout of v_mfma_f32_16x16x32_bf16: lane l, item r -> row 4 (l / 16) + r, column l % 16
   float* w = ws + (int64_t)ks * M * npad;

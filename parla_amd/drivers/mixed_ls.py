@@ -102,7 +102,7 @@ class SPOMixed(OverLstsqSolver):
         else:
             Q = ops.gauss_q(d, m, int(rng.integers(0, 2 ** 63)), dev)
         split = ops.OzakiSplit(At, bt, self.npieces)
-        Mx = ops.ozaki_sketch(Q, split, 2.0 ** -5 / np.sqrt(min(d, m)))
+        Mx = ops.ozaki_sketch(Q, d, split, 2.0 ** -5 / np.sqrt(min(d, m)))
         del split, Q
         log.time_sketch = clock() - tic
 

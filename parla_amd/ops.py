@@ -441,29 +441,52 @@ def _round_up(x, q):
     return ((x + q - 1) // q) * q
 
 
+def ozaki_layout(d, m, ncols):
+    """(kp, dpad, ncp, group_chunks, workspace bytes) of the exact sketch."""
+    v = [ctypes.c_int64() for _ in range(5)]
+    check(lib().pamd_ozaki_layout(d, m, ncols, *[ctypes.byref(x) for x in v]), "pamd_ozaki_layout")
+    return tuple(x.value for x in v)
+
+
 def gauss_q(rows, m, seed, dev, offset=0):
-    """Quantised Gaussian Q (rows x kp int-valued BF16, kp = m rounded up to
-    256, zeros beyond m): Q = rint(32 z) for the Philox stream of gaussian_fill."""
+    """Quantised Gaussian Q = rint(32 z) for the Philox stream of
+    gaussian_fill, as a blocked BF16 array (rows rounded up to 128 x kp, kp
+    = m rounded up to 256, zeros beyond); `unblock_pieces(Q[None])` gives the
+    row-major view."""
     kp = _round_up(max(m, 1), OZ_CHUNK)
-    Q = torch.empty((rows, kp), dtype=torch.bfloat16, device=dev)
-    check(lib().pamd_gauss_q_bf16(rows, m, seed & (2 ** 64 - 1), offset, _p(Q), kp, stream(dev)),
+    rows_out = _round_up(max(rows, 1), 128)
+    Q = torch.empty((rows_out, kp), dtype=torch.uint16, device=dev)
+    check(lib().pamd_gauss_q_bf16(rows, rows_out, m, seed & (2 ** 64 - 1), offset, _p(Q), stream(dev)),
           "pamd_gauss_q_bf16")
     return Q
 
 
+def block_rows(X, rows_out, kp):
+    """Blocked layout (blocked.h) of a row-major 2-D device tensor of 16-bit
+    values, zero-padded to rows_out x kp: the device-side form of
+    block_pieces for one array."""
+    R0, K0 = X.shape
+    Xp = torch.zeros((rows_out, kp), dtype=torch.int16, device=X.device)  # indexing has no uint16 kernel
+    Xp[:R0, :K0] = X.view(torch.int16)
+    b = Xp.view(rows_out // 16, 16, kp // 32, 4, 8).permute(0, 2, 1, 3, 4)
+    r = torch.arange(16, device=X.device)[:, None]
+    logical = torch.arange(4, device=X.device)[None, :] ^ (r >> 2)
+    return b[:, :, r, logical, :].contiguous().view(rows_out, kp).view(X.dtype)
+
+
 def quantize_q(Z, m=None):
-    """Q = clip(rint(32 Z), -256, 256) as a chunk-padded BF16 tensor (the host
-    draw of gauss_q: numpy normals -> the same integer grid)."""
+    """Q = clip(rint(32 Z), -256, 256) in the blocked BF16 layout of gauss_q
+    (the host draw of gauss_q: numpy normals -> the same integer grid)."""
     Z = as_device(Z)
     rows, mm = Z.shape
     kp = _round_up(max(mm, 1), OZ_CHUNK)
-    Q = torch.zeros((rows, kp), dtype=torch.bfloat16, device=Z.device)
-    Q[:, :mm] = torch.clamp(torch.round(32.0 * Z), -256.0, 256.0).to(torch.bfloat16)
-    return Q
+    q = torch.clamp(torch.round(32.0 * Z), -256.0, 256.0).to(torch.bfloat16).view(torch.uint16)
+    return block_rows(q, _round_up(max(rows, 1), 128), kp)
 
 
 class OzakiSplit:
-    """[A | b] cut into `npieces` BF16 integer pieces per 256-row chunk."""
+    """[A | b] cut into `npieces` BF16 integer pieces per 256-row chunk
+    (blocked arrays of ncp x kp, one per piece)."""
 
     def __init__(self, A, b=None, npieces=4):
         m, n = A.shape
@@ -472,31 +495,36 @@ class OzakiSplit:
         self.m, self.n, self.npieces = m, n, npieces
         self.ncols = n + (1 if b is not None else 0)
         self.kp = _round_up(max(m, 1), OZ_CHUNK)
-        self.ncp = _round_up(self.ncols, OZ_TN)
+        self.ncp = _round_up(max(self.ncols, 1), OZ_TN)
         self.pstride = self.ncp * self.kp
         dev = A.device
-        self.pieces = torch.empty(npieces * self.pstride, dtype=torch.bfloat16, device=dev)
+        self.pieces = torch.empty((npieces, self.ncp, self.kp), dtype=torch.uint16, device=dev)
         self.exps = torch.empty((self.kp // OZ_CHUNK, self.ncols), dtype=torch.int32, device=dev)
         bb = None if b is None else b.contiguous()
         check(lib().pamd_ozaki_split_bf16(m, n, _p(A), max(A.stride(0), n, 1), _p(bb), npieces, _p(self.pieces),
-                                          self.kp, self.pstride, _p(self.exps), self.ncols, stream(dev)),
+                                          self.pstride, _p(self.exps), self.ncols, stream(dev)),
               "pamd_ozaki_split_bf16")
 
     def piece(self, p):
-        """Piece p as an (ncols x kp) tensor (column j's K run in row j)."""
-        return self.pieces[p * self.pstride:(p + 1) * self.pstride].view(self.ncp, self.kp)[:self.ncols]
+        """Piece p as a row-major numpy array (ncols x kp, BF16 values as float32)."""
+        bits = unblock_pieces(self.pieces[p:p + 1])[0, :self.ncols]
+        return (bits.astype(np.uint32) << 16).view(np.float32)
 
 
-def ozaki_sketch(Q, split, scale, out=None):
-    """out (d x ncols) = scale * Q @ [A | b] from the exact BF16 products."""
-    d = Q.shape[0]
+def ozaki_sketch(Q, d, split, scale, out=None):
+    """out (d x ncols) = scale * Q[:d] @ [A | b] from the exact BF16 products
+    (Q from gauss_q / quantize_q, d its logical rows)."""
+    kp, dpad, ncp, _, wsb = ozaki_layout(d, split.m, split.ncols)
+    if Q.shape[0] < dpad or Q.shape[1] != kp or split.kp != kp:
+        raise ValueError("ozaki_sketch: Q does not match the layout")
     if out is None:
         out = torch.empty((d, split.ncols), dtype=F64, device=Q.device)
-    with _Timed("ozaki_sketch", 2.0 * d * split.kp * split.ncp * split.npieces,
-                2.0 * d * split.kp + 2.0 * split.npieces * split.kp * split.ncp + 8.0 * d * split.ncols):
-        check(lib().pamd_ozaki_sketch_f64(d, split.m, split.ncols, split.npieces, _p(Q), Q.stride(0),
-                                          _p(split.pieces), split.kp, split.pstride, _p(split.exps), split.ncols,
-                                          float(scale), _p(out), out.stride(0), stream(Q.device)),
+    ws = workspace(wsb, Q.device)
+    with _Timed("ozaki_sketch", 2.0 * dpad * kp * ncp * split.npieces,
+                2.0 * dpad * kp + 2.0 * split.npieces * kp * ncp + 8.0 * d * split.ncols):
+        check(lib().pamd_ozaki_sketch_f64(d, split.m, split.ncols, split.npieces, _p(Q), _p(split.pieces),
+                                          split.pstride, _p(split.exps), split.ncols, float(scale), _p(out),
+                                          out.stride(0), _p(ws), stream(Q.device)),
               "pamd_ozaki_sketch_f64")
     return out
 

@@ -32,9 +32,12 @@ def test_gauss_q_matches_fill_quantised(ops):
     rows, m, seed = 37, 1000, 1234
     Q = ops.gauss_q(rows, m, seed, torch.device("cuda"))
     Z = ops.gaussian_fill(rows, m, 1.0, seed, torch.device("cuda"))
-    Qh = Q.float().cpu().numpy()
-    assert Qh.shape[1] == 1024 and np.all(Qh[:, m:] == 0)
-    np.testing.assert_array_equal(Qh[:, :m], orc.quantize_q(Z.cpu().numpy()))
+    bits = ops.unblock_pieces(Q[None])[0]
+    Qh = (bits.astype(np.uint32) << 16).view(np.float32)
+    assert Qh.shape == (128, 1024) and np.all(Qh[:, m:] == 0) and np.all(Qh[rows:] == 0)
+    np.testing.assert_array_equal(Qh[:rows, :m], orc.quantize_q(Z.cpu().numpy()))
+    # the host draw's blocked layout is the same array
+    assert torch.equal(ops.quantize_q(Z), Q)
 
 
 @pytest.mark.parametrize("m,n,with_b,P", [(700, 30, True, 4), (256, 64, False, 3), (1000, 65, True, 5),
@@ -48,7 +51,7 @@ def test_ozaki_split_bit_exact(ops, m, n, with_b, P):
     pieces, exps = orc.ozaki_split(A, b, P)
     np.testing.assert_array_equal(sp.exps.cpu().numpy(), exps)
     for p in range(P):
-        got = sp.piece(p).float().cpu().numpy()[:, :m].T
+        got = sp.piece(p)[:, :m].T
         np.testing.assert_array_equal(got, pieces[p])
 
 
@@ -64,9 +67,10 @@ def test_ozaki_sketch_exact(ops, d, m, n, P):
     Q = ops.quantize_q(_dev(Z))
     sp = ops.OzakiSplit(_dev(A), _dev(b), P)
     scale = 2.0 ** -5 / np.sqrt(min(d, m))
-    out = ops.ozaki_sketch(Q, sp, scale).cpu().numpy()
+    out = ops.ozaki_sketch(Q, d, sp, scale).cpu().numpy()
     pieces, exps = orc.ozaki_split(A, b, P)
-    ref = orc.ozaki_product(orc.quantize_q(Z), pieces, exps, scale)
+    group = ops.ozaki_layout(d, m, n + 1)[3]
+    ref = orc.ozaki_product(orc.quantize_q(Z), pieces, exps, scale, group_chunks=group)
     np.testing.assert_array_equal(out, ref)
     # and it is the sketch of [A | b] to the truncation level 2^-9P
     full = scale * orc.quantize_q(Z) @ np.column_stack([A, b])
@@ -148,3 +152,29 @@ def test_spo_mixed_reaches_fp64_floor(ops, cond):
     assert np.linalg.norm(x - xl) <= 1e-15 * cond * 100 * np.linalg.norm(xl)
     xo, eo = orc.spo_mixed(A, b, 1e-12, 5)
     assert np.linalg.norm(x - xo) <= 1e-15 * cond * 100 * np.linalg.norm(xl)
+
+
+def test_ozaki_sketch_full_size_long_groups(ops):
+    """At the 262144 x 1024 shape every workgroup streams 1024 K steps: the
+    result is reproducible bit for bit and within the truncation bound of the
+    FP64 product everywhere (a stale LDS buffer shows up as a piece-sized
+    error in some entries and a run-to-run difference)."""
+    d, m, n, P = 4096, 262144, 1024, 4
+    _say(f"ozaki sketch full size d={d} m={m} n={n} P={P}")
+    dev = torch.device("cuda")
+    assert ops.ozaki_layout(d, m, n + 1)[3] == 128  # chunks per K group
+    Q = ops.gauss_q(d, m, 99, dev)
+    Qf = torch.clamp(torch.round(32.0 * ops.gaussian_fill(d, m, 1.0, 99, dev)), -256.0, 256.0)
+    g = torch.Generator(device=dev).manual_seed(3)
+    A = torch.randn((m, n), generator=g, dtype=torch.float64, device=dev)
+    b = torch.randn((m,), generator=g, dtype=torch.float64, device=dev)
+    sp = ops.OzakiSplit(A, b, P)
+    scale = 2.0 ** -5 / np.sqrt(d)
+    out1 = ops.ozaki_sketch(Q, d, sp, scale)
+    out2 = ops.ozaki_sketch(Q, d, sp, scale)
+    assert torch.equal(out1, out2)
+    Ab = torch.cat([A, b[:, None]], dim=1)
+    full = scale * (Qf @ Ab)
+    bound = scale * Qf.abs().sum(dim=1, keepdim=True) * Ab.abs().max()
+    bad = int(((out1 - full).abs() > 2.0 ** (-9 * P) * bound + 1e-13 * full.abs().max()).sum().item())
+    assert bad == 0, bad

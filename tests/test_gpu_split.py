@@ -118,3 +118,23 @@ def test_qb_with_split_operator(ops, P):
     assert torch.linalg.norm(Q.t() @ Q - I).item() <= 1e-13
     Bex = (Q.t() @ A)
     assert torch.linalg.norm(B - Bex).item() <= 2.0 ** -(8 * P - 3) * fro
+
+
+@pytest.mark.parametrize("P", [2, 3])
+def test_split_gemm_long_k_reproducible(ops, P):
+    """32768 K per workgroup (4 K splits of 131072): bit-for-bit reproducible
+    and within the FP32-accumulation bound of the FP64 product."""
+    M, N, K = 8192, 266, 131072
+    _say(f"split gemm long K P={P} M={M} N={N} K={K}")
+    g = torch.Generator(device="cuda").manual_seed(P)
+    L = torch.randn((M, K), generator=g, dtype=torch.float64, device="cuda")
+    R = torch.randn((N, K), generator=g, dtype=torch.float64, device="cuda")
+    kp, mpad, npad, _ = ops.split_layout(P, M, N, K)
+    Lp = ops.split_bf16(L, P)
+    Rp = ops.split_bf16(R, P, rows_out=npad)
+    C1 = ops.split_gemm(Lp, Rp)
+    C2 = ops.split_gemm(Lp, Rp)
+    assert torch.equal(C1, C2)
+    ref = L @ R.t()
+    err = (torch.linalg.norm(C1 - ref) / torch.linalg.norm(ref)).item()
+    assert err <= 4 * np.sqrt(K) * 2.0 ** -24 + 2.0 ** -(8 * P - 4), err

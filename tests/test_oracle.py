@@ -372,6 +372,8 @@ def test_ozaki_product_is_the_sketch_of_the_truncated_matrix():
     Q = orc.quantize_q(rng.standard_normal((d, m)))
     pieces, exps = orc.ozaki_split(A, None, P)
     out = orc.ozaki_product(Q, pieces, exps, 1.0)
+    grouped = orc.ozaki_product(Q, pieces, exps, 1.0, group_chunks=1)
+    assert np.array_equal(orc.ozaki_product(Q, pieces, exps, 1.0, group_chunks=3), out)
     for i in range(d):
         for j in range(n):
             tot = Fraction(0)
@@ -380,6 +382,7 @@ def test_ozaki_product_is_the_sketch_of_the_truncated_matrix():
                 at = sum(Fraction(int(pieces[p, k, j])) * Fraction(2) ** int(exps[c, j] - 8 - 9 * p) for p in range(P))
                 tot += int(Q[i, k]) * at
             assert abs(float(tot) - out[i, j]) <= 4 * np.finfo(float).eps * abs(float(tot))
+            assert abs(float(tot) - grouped[i, j]) <= 4 * np.finfo(float).eps * abs(float(tot))
 
 
 def test_residual_exact_against_exact_rationals():

@@ -36,3 +36,15 @@ def test_swizzle_makes_fragment_reads_conflict_free():
             byte = r * 64 + 16 * (c ^ (r // 4))
             banks.add((byte // 4) % 64 // 4)
         assert len(banks) == 16
+
+
+def test_block_rows_pads_and_matches_block_pieces():
+    """The tensor form (used for the quantised Gaussian) pads with zeros and
+    lays out exactly as the numpy form."""
+    import torch
+    rng = np.random.default_rng(1)
+    X = rng.integers(0, 65535, (37, 200)).astype(np.uint16)
+    got = ops.block_rows(torch.from_numpy(X), 128, 256).numpy()
+    padded = np.zeros((1, 128, 256), dtype=np.uint16)
+    padded[0, :37, :200] = X
+    assert np.array_equal(got, ops.block_pieces(padded).numpy()[0])
