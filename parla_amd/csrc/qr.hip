@@ -18,7 +18,7 @@
 namespace {
 
 constexpr int QR_NB = 8;
-constexpr int QR_CHUNK = 256;  // rows per partial in V^T C
+constexpr int QR_CHUNK = 64;  // rows per partial in V^T C (16 per wavefront, all loads in flight)
 
 struct QrLayout {
   int64_t npan;       // number of panels
@@ -49,19 +49,64 @@ QrLayout qr_layout(int64_t m, int64_t n, int64_t ncols) {
   return L;
 }
 
+
+constexpr int QR_NWAVES = 16;  // panel kernels run 1024 threads
+constexpr int QR_NPAIR = QR_NB * (QR_NB - 1) / 2;
+
+// The compact-WY factor T (LAPACK dlarft, forward / columnwise) from the
+// Gram entries Gp of the panel's V columns (pairs a < b in row-major order):
+// T[i, j] = -tau_j sum_{i <= l < j} T[i, l] G[l, j].  Row i of T needs only
+// row i, so thread i forms it.  Ends with T and tau written.
+__device__ __forceinline__ void qr_panel_T(const double* Gp, const double* taus, int nb, double* __restrict__ T,
+                                           double* __restrict__ tau_out, int tid) {
+  if (tid < QR_NB) {
+    const int i = tid;
+    double Trow[QR_NB];
+#pragma unroll
+    for (int b = 0; b < QR_NB; ++b) Trow[b] = 0.0;
+    if (i < nb) Trow[i] = taus[i];
+#pragma unroll
+    for (int jj = 1; jj < QR_NB; ++jj) {
+      if (jj < nb && jj > i) {
+        double sum = 0.0;
+#pragma unroll
+        for (int l = 0; l < jj; ++l)
+          if (l >= i) sum += Trow[l] * Gp[l * (2 * QR_NB - l - 1) / 2 + (jj - l - 1)];
+        Trow[jj] = -taus[jj] * sum;
+      }
+    }
+    tau_out[i] = (i < nb) ? taus[i] : 0.0;
+#pragma unroll
+    for (int b = 0; b < QR_NB; ++b) T[i * QR_NB + b] = Trow[b];
+  }
+}
+
+// After red[w][q] holds wave w's partial of the Gram entry q (1024 threads):
+// G by QR_NPAIR threads (waves summed in order), then T.
+__device__ __forceinline__ void qr_panel_tail(double (*red)[QR_NPAIR + 1], double* Gp, const double* taus, int nb,
+                                              double* __restrict__ T, double* __restrict__ tau_out, int tid) {
+  if (tid < QR_NPAIR) {
+    double sum = 0.0;
+#pragma unroll
+    for (int w = 0; w < QR_NWAVES; ++w) sum += red[w][tid];
+    Gp[tid] = sum;
+  }
+  __syncthreads();
+  qr_panel_T(Gp, taus, nb, T, tau_out, tid);
+}
+
 // Factor columns [c0, c0 + nb) of rows [c0, m) (nb <= QR_NB).
 __global__ __launch_bounds__(1024) void qr_panel_kernel(int64_t m, int64_t c0, int nb, double* __restrict__ A,
                                                         int64_t lda, double* __restrict__ V,
                                                         double* __restrict__ T, double* __restrict__ tau_out,
                                                         int* __restrict__ singular) {
-  __shared__ double red[16][QR_NB * (QR_NB + 1) / 2 + 1];
+  __shared__ double red[QR_NWAVES][QR_NPAIR + 1];
+  __shared__ double tots[QR_NB];
   __shared__ double taus[QR_NB];
-  __shared__ double G[QR_NB][QR_NB];
-  __shared__ double Tl[QR_NB][QR_NB];
+  __shared__ double Gp[QR_NPAIR];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int nwaves = blockDim.x >> 6;
 
   for (int j = 0; j < nb; ++j) {
     const int64_t pj = c0 + j;
@@ -83,13 +128,16 @@ __global__ __launch_bounds__(1024) void qr_panel_kernel(int64_t m, int64_t c0, i
       if (lane == 0) red[wave][c] = s;
     }
     __syncthreads();
+    if (tid < QR_NB) {  // one lane per value sums the waves in order
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < QR_NWAVES; ++w) s += red[w][tid];
+      tots[tid] = s;
+    }
+    __syncthreads();
     double tot[QR_NB];
 #pragma unroll
-    for (int c = 0; c < QR_NB; ++c) {
-      double s = 0.0;
-      for (int w = 0; w < nwaves; ++w) s += red[w][c];
-      tot[c] = s;
-    }
+    for (int c = 0; c < QR_NB; ++c) tot[c] = tots[c];
     double* prow = A + pj * lda + c0;
     const double alpha = prow[j];
     double beta, tau, scale;
@@ -142,10 +190,9 @@ __global__ __launch_bounds__(1024) void qr_panel_kernel(int64_t m, int64_t c0, i
   }
   __syncthreads();
   // G = V^T V (upper part, pairs a < b), reduced in fixed order
-  constexpr int NPAIR = QR_NB * (QR_NB - 1) / 2;
-  double g[NPAIR];
+  double g[QR_NPAIR];
 #pragma unroll
-  for (int q = 0; q < NPAIR; ++q) g[q] = 0.0;
+  for (int q = 0; q < QR_NPAIR; ++q) g[q] = 0.0;
   for (int64_t r = tid; r < mr; r += blockDim.x) {
     double vr[QR_NB];
 #pragma unroll
@@ -157,78 +204,103 @@ __global__ __launch_bounds__(1024) void qr_panel_kernel(int64_t m, int64_t c0, i
       for (int b = a + 1; b < QR_NB; ++b) g[q++] += vr[a] * vr[b];
   }
 #pragma unroll
-  for (int q = 0; q < NPAIR; ++q) {
+  for (int q = 0; q < QR_NPAIR; ++q) {
     const double s = wave_sum(g[q]);
     if (lane == 0) red[wave][q] = s;
   }
   __syncthreads();
-  if (tid == 0) {
-    int q = 0;
-    for (int a = 0; a < QR_NB; ++a)
-      for (int b = a + 1; b < QR_NB; ++b) {
-        double s = 0.0;
-        for (int w = 0; w < nwaves; ++w) s += red[w][q];
-        G[a][b] = s;
-        ++q;
-      }
-    // dlarft forward columnwise: T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j]
-    for (int a = 0; a < QR_NB; ++a)
-      for (int b = 0; b < QR_NB; ++b) Tl[a][b] = 0.0;
-    for (int jj = 0; jj < nb; ++jj) {
-      const double tj = taus[jj];
-      Tl[jj][jj] = tj;
-      for (int i = 0; i < jj; ++i) {
-        double s = 0.0;
-        for (int l = i; l < jj; ++l) s += Tl[i][l] * G[l][jj];
-        Tl[i][jj] = -tj * s;
-      }
-    }
-    for (int a = 0; a < QR_NB; ++a) {
-      tau_out[a] = (a < nb) ? taus[a] : 0.0;
-      for (int b = 0; b < QR_NB; ++b) T[a * QR_NB + b] = Tl[a][b];
-    }
-  }
+  qr_panel_tail(red, Gp, taus, nb, T, tau_out, tid);
 }
 
-// Same factorisation with the panel held in registers: thread t owns the panel
-// rows t + 1024 q (q < RPT), so each column costs one fused reduction (sum of
-// squares + dot products, 8 values) and no memory traffic until the end.
+// Wave-wide sums of N values at once (N a power of two, <= 64): in round k
+// each lane swaps half of its remaining values with lane ^ 2^k and keeps the
+// half selected by bit k of its lane, so log2 N rounds leave one value per
+// lane, and the remaining rounds add it across the wave.  Returns, in lane l,
+// the wave total of v[idx], idx = sum_k bit_k(l) N / 2^(k+1) (the low log2 N
+// bits of l reversed); every lane with the same low bits holds the same
+// total.  About 2N - 1 + 6 - log2 N shuffles instead of 6 N.
+template <int N>
+__device__ __forceinline__ double wave_sum_multi(double (&v)[N], int lane) {
+  constexpr int LOG2N = (N >= 64) ? 6 : (N >= 32) ? 5 : (N >= 16) ? 4 : (N >= 8) ? 3 : (N >= 4) ? 2 : (N >= 2) ? 1 : 0;
+  static_assert((1 << LOG2N) == N && N <= 64, "N a power of two, at most 64");
+#pragma unroll
+  for (int round = 0; round < LOG2N; ++round) {
+    const bool hi = (lane >> round) & 1;
+#pragma unroll
+    for (int i = 0; i < (N >> (round + 1)); ++i) {
+      const int h = N >> (round + 1);
+      // both values in registers before the selects (a select of the two
+      // addresses would keep v in scratch memory)
+      double lo = v[i], up = v[i + h];
+      asm volatile("" : "+v"(lo), "+v"(up));
+      const double send = hi ? lo : up;
+      const double keep = hi ? up : lo;
+      v[i] = keep + __shfl_xor(send, 1 << round, 64);
+    }
+  }
+  double t = v[0];
+#pragma unroll
+  for (int m = N; m < 64; m <<= 1) t += __shfl_xor(t, m, 64);
+  return t;
+}
+
+template <int N>
+__device__ __forceinline__ int wave_sum_multi_index(int lane) {
+  int idx = 0;
+#pragma unroll
+  for (int k = 0; (N >> k) > 1; ++k) idx += ((lane >> k) & 1) * (N >> (k + 1));
+  return idx;
+}
+
+// Same factorisation with the panel held in registers: 512 threads, thread t
+// owns the panel rows t + 512 q (q < RPT).  Each column costs one fused
+// reduction (sum of squares + dot products, 8 values) and two barriers; the
+// shared values of a column are double-buffered by its parity, so no barrier
+// is needed between columns.  No memory traffic until the end.
+constexpr int QR_REG_THREADS = 512;
+
 template <int RPT>
-__global__ __launch_bounds__(1024) void qr_panel_reg_kernel(int64_t m, int64_t c0, int nb, double* __restrict__ A,
-                                                            int64_t lda, double* __restrict__ V,
-                                                            double* __restrict__ T, double* __restrict__ tau_out,
-                                                            int* __restrict__ singular) {
-  __shared__ double red[16][QR_NB * (QR_NB - 1) / 2 + 1];
-  __shared__ double prow[QR_NB];
-  __shared__ double tots[QR_NB];
+__global__ __launch_bounds__(QR_REG_THREADS) void qr_panel_reg_kernel(int64_t m, int64_t c0, int nb,
+                                                                      double* __restrict__ A, int64_t lda,
+                                                                      double* __restrict__ V,
+                                                                      double* __restrict__ T,
+                                                                      double* __restrict__ tau_out,
+                                                                      int* __restrict__ singular) {
+  constexpr int NT = QR_REG_THREADS, NW = NT / 64;
+  constexpr int NG = 32;  // Gram pairs padded to a power of two
+  static_assert(QR_NPAIR <= NG, "pairs fit the padded reduction");
+  __shared__ double red[2][NW][QR_NB];
+  __shared__ double prow[2][QR_NB];
+  __shared__ double tots[2][QR_NB];
   __shared__ double taus[QR_NB];
-  __shared__ double G[QR_NB][QR_NB];
-  __shared__ double Tl[QR_NB][QR_NB];
+  __shared__ double gred[NW][NG];
+  __shared__ double Gp[QR_NPAIR];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int nwaves = blockDim.x >> 6;
   const int64_t mr = m - c0;
   double a[RPT][QR_NB];
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
-    const int64_t r = tid + 1024 * q;
+    const int64_t r = tid + NT * q;
 #pragma unroll
     for (int c = 0; c < QR_NB; ++c) a[q][c] = (r < mr && c < nb) ? A[(c0 + r) * lda + c0 + c] : 0.0;
   }
+  const int idx8 = wave_sum_multi_index<QR_NB>(lane);
 #pragma unroll
   for (int j = 0; j < QR_NB; ++j) {  // unrolled: every register index is static
     if (j >= nb) break;
+    const int buf = j & 1;
     if (tid == j) {
 #pragma unroll
-      for (int c = 0; c < QR_NB; ++c) prow[c] = a[0][c];
+      for (int c = 0; c < QR_NB; ++c) prow[buf][c] = a[0][c];
     }
     double part[QR_NB];
 #pragma unroll
     for (int c = 0; c < QR_NB; ++c) part[c] = 0.0;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      const int64_t r = tid + 1024 * q;
+      const int64_t r = tid + NT * q;
       if (r > j && r < mr) {
         const double x = a[q][j];
         part[0] += x * x;
@@ -237,22 +309,20 @@ __global__ __launch_bounds__(1024) void qr_panel_reg_kernel(int64_t m, int64_t c
           if (c > j) part[c] += x * a[q][c];
       }
     }
-#pragma unroll
-    for (int c = 0; c < QR_NB; ++c) {
-      const double sum = wave_sum(part[c]);
-      if (lane == 0) red[wave][c] = sum;
-    }
+    const double wsum = wave_sum_multi<QR_NB>(part, lane);
+    if (lane < QR_NB) red[buf][wave][idx8] = wsum;
     __syncthreads();
     if (tid < QR_NB) {  // one lane per value sums the waves in order
       double sum = 0.0;
-      for (int w = 0; w < nwaves; ++w) sum += red[w][tid];
-      tots[tid] = sum;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) sum += red[buf][w][tid];
+      tots[buf][tid] = sum;
     }
     __syncthreads();
     double tot[QR_NB];
 #pragma unroll
-    for (int c = 0; c < QR_NB; ++c) tot[c] = tots[c];
-    const double alpha = prow[j];
+    for (int c = 0; c < QR_NB; ++c) tot[c] = tots[buf][c];
+    const double alpha = prow[buf][j];
     double beta, tau, scale;
     if (tot[0] == 0.0) {
       beta = alpha;
@@ -265,10 +335,10 @@ __global__ __launch_bounds__(1024) void qr_panel_reg_kernel(int64_t m, int64_t c
     }
     double w[QR_NB];
 #pragma unroll
-    for (int c = 0; c < QR_NB; ++c) w[c] = (c > j && c < nb) ? tau * (prow[c] + scale * tot[c]) : 0.0;
+    for (int c = 0; c < QR_NB; ++c) w[c] = (c > j && c < nb) ? tau * (prow[buf][c] + scale * tot[c]) : 0.0;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      const int64_t r = tid + 1024 * q;
+      const int64_t r = tid + NT * q;
       if (r > j && r < mr) {
         const double v = a[q][j] * scale;
         a[q][j] = v;
@@ -285,16 +355,15 @@ __global__ __launch_bounds__(1024) void qr_panel_reg_kernel(int64_t m, int64_t c
       taus[j] = tau;
       if (beta == 0.0) singular[0] = 1;
     }
-    __syncthreads();  // prow / red are rewritten by the next column
   }
-  // write back: R above the diagonal, v below; V with explicit unit diagonal
-  constexpr int NPAIR = QR_NB * (QR_NB - 1) / 2;
-  double g[NPAIR];
+  // write back: R above the diagonal, v below; V with explicit unit diagonal;
+  // Gram partials of V's columns (pairs x < y) on the way
+  double g[NG];
 #pragma unroll
-  for (int p = 0; p < NPAIR; ++p) g[p] = 0.0;
+  for (int p = 0; p < NG; ++p) g[p] = 0.0;
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
-    const int64_t r = tid + 1024 * q;
+    const int64_t r = tid + NT * q;
     if (r < mr) {
       double vr[QR_NB];
 #pragma unroll
@@ -310,58 +379,49 @@ __global__ __launch_bounds__(1024) void qr_panel_reg_kernel(int64_t m, int64_t c
         for (int y = x + 1; y < QR_NB; ++y) g[p++] += vr[x] * vr[y];
     }
   }
+  const double gsum = wave_sum_multi<NG>(g, lane);
+  if (lane < NG) gred[wave][wave_sum_multi_index<NG>(lane)] = gsum;
+  __syncthreads();  // also: taus of every column are written
+  if (tid < QR_NPAIR) {
+    double sum = 0.0;
 #pragma unroll
-  for (int p = 0; p < NPAIR; ++p) {
-    const double sum = wave_sum(g[p]);
-    if (lane == 0) red[wave][p] = sum;
+    for (int w = 0; w < NW; ++w) sum += gred[w][tid];
+    Gp[tid] = sum;
   }
   __syncthreads();
-  if (tid == 0) {
-    int p = 0;
-    for (int x = 0; x < QR_NB; ++x)
-      for (int y = x + 1; y < QR_NB; ++y) {
-        double sum = 0.0;
-        for (int w = 0; w < nwaves; ++w) sum += red[w][p];
-        G[x][y] = sum;
-        ++p;
-      }
-    for (int x = 0; x < QR_NB; ++x)
-      for (int y = 0; y < QR_NB; ++y) Tl[x][y] = 0.0;
-    for (int jj = 0; jj < nb; ++jj) {
-      const double tj = taus[jj];
-      Tl[jj][jj] = tj;
-      for (int i = 0; i < jj; ++i) {
-        double sum = 0.0;
-        for (int l = i; l < jj; ++l) sum += Tl[i][l] * G[l][jj];
-        Tl[i][jj] = -tj * sum;
-      }
-    }
-    for (int x = 0; x < QR_NB; ++x) {
-      tau_out[x] = (x < nb) ? taus[x] : 0.0;
-      for (int y = 0; y < QR_NB; ++y) T[x * QR_NB + y] = Tl[x][y];
-    }
-  }
+  qr_panel_T(Gp, taus, nb, T, tau_out, tid);
 }
 
-// Wpart[chunk][r][c] = sum_{i in chunk} V[i][r] * C[i][c]; C = A[rows0.., cols0..]
+// Wpart[chunk][r][c] = sum_{i in chunk} V[i][r] * C[i][c]; C = A[rows0.., cols0..].
+// Wavefront w sums the chunk's rows w, w + 4, ... in order (its 16 loads of C
+// issued together); the four wavefronts are added in order.
 __global__ __launch_bounds__(256) void qr_vtc_kernel(int64_t mr, int64_t nc, const double* __restrict__ V,
                                                      const double* __restrict__ C, int64_t ldc,
                                                      double* __restrict__ Wpart) {
+  constexpr int RW = QR_CHUNK / 4;  // rows per wavefront
   __shared__ double red[4][QR_NB][64];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t col = (int64_t)blockIdx.x * 64 + lane;
   const int64_t i0 = (int64_t)blockIdx.y * QR_CHUNK;
-  const int64_t i1 = min(mr, i0 + QR_CHUNK);
   double acc[QR_NB];
 #pragma unroll
   for (int r = 0; r < QR_NB; ++r) acc[r] = 0.0;
   if (col < nc) {
-    for (int64_t i = i0 + wave; i < i1; i += 4) {
-      const double c = C[i * ldc + col];
-      const double* v = V + i * QR_NB;
+    double c[RW];
 #pragma unroll
-      for (int r = 0; r < QR_NB; ++r) acc[r] += v[r] * c;
+    for (int u = 0; u < RW; ++u) {
+      const int64_t i = i0 + wave + 4 * u;
+      c[u] = (i < mr) ? C[i * ldc + col] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < RW; ++u) {
+      const int64_t i = i0 + wave + 4 * u;
+      if (i < mr) {
+        const double* v = V + i * QR_NB;
+#pragma unroll
+        for (int r = 0; r < QR_NB; ++r) acc[r] += v[r] * c[u];
+      }
     }
   }
 #pragma unroll
@@ -375,24 +435,35 @@ __global__ __launch_bounds__(256) void qr_vtc_kernel(int64_t mr, int64_t nc, con
   }
 }
 
-// W2[r][c] = sum_r' op(T)[r][r'] * (sum_chunks Wpart[chunk][r'][c]); op = T^T (apply Q^T) or T (apply Q)
-__global__ void qr_w2_kernel(int64_t nc, int nchunks, const double* __restrict__ Wpart,
-                             const double* __restrict__ T, int transT, double* __restrict__ W2) {
-  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= nc) return;
-  double w[QR_NB];
+// W2[r][c] = sum_r' op(T)[r][r'] * (sum_chunks Wpart[chunk][r'][c]); op = T^T (apply Q^T) or T (apply Q).
+// Thread (r, c) of a 32-column block sums its chunk partials in chunk order
+// (eight loads in flight), then the block applies op(T) through LDS.
+__global__ __launch_bounds__(256) void qr_w2_kernel(int64_t nc, int nchunks, const double* __restrict__ Wpart,
+                                                    const double* __restrict__ T, int transT,
+                                                    double* __restrict__ W2) {
+  __shared__ double sw[QR_NB][33];
+  const int cl = threadIdx.x & 31, r = threadIdx.x >> 5;
+  const int64_t col = (int64_t)blockIdx.x * 32 + cl;
+  double w = 0.0;
+  if (col < nc) {
+    const double* p = Wpart + (int64_t)r * nc + col;
+    const int64_t cs = (int64_t)QR_NB * nc;
+    int ch = 0;
+    for (; ch + 8 <= nchunks; ch += 8) {
+      double x[8];
 #pragma unroll
-  for (int r = 0; r < QR_NB; ++r) w[r] = 0.0;
-  for (int ch = 0; ch < nchunks; ++ch) {
-    const double* p = Wpart + (int64_t)ch * QR_NB * nc + col;
+      for (int u = 0; u < 8; ++u) x[u] = p[(int64_t)(ch + u) * cs];
 #pragma unroll
-    for (int r = 0; r < QR_NB; ++r) w[r] += p[r * nc];
+      for (int u = 0; u < 8; ++u) w += x[u];
+    }
+    for (; ch < nchunks; ++ch) w += p[(int64_t)ch * cs];
   }
-#pragma unroll
-  for (int r = 0; r < QR_NB; ++r) {
+  sw[r][cl] = w;
+  __syncthreads();
+  if (col < nc) {
     double s = 0.0;
 #pragma unroll
-    for (int q = 0; q < QR_NB; ++q) s += (transT ? T[q * QR_NB + r] : T[r * QR_NB + q]) * w[q];
+    for (int q = 0; q < QR_NB; ++q) s += (transT ? T[q * QR_NB + r] : T[r * QR_NB + q]) * sw[q][cl];
     W2[r * nc + col] = s;
   }
 }
@@ -436,7 +507,7 @@ int apply_block(int64_t mr, int64_t nc, const double* V, const double* T, int tr
   dim3 g1((unsigned)pamd_cdiv(nc, 64), (unsigned)nchunks);
   qr_vtc_kernel<<<g1, 256, 0, st>>>(mr, nc, V, C, ldc, Wpart);
   PAMD_LAUNCH_CHECK();
-  qr_w2_kernel<<<(unsigned)pamd_cdiv(nc, 256), 256, 0, st>>>(nc, nchunks, Wpart, T, transT, W2);
+  qr_w2_kernel<<<(unsigned)pamd_cdiv(nc, 32), 256, 0, st>>>(nc, nchunks, Wpart, T, transT, W2);
   PAMD_LAUNCH_CHECK();
   dim3 g3((unsigned)pamd_cdiv(nc, 64), (unsigned)pamd_cdiv(mr, 16));
   qr_update_kernel<<<g3, 256, 0, st>>>(mr, nc, V, W2, C, ldc);
@@ -477,12 +548,15 @@ extern "C" int pamd_geqrf_f64(int64_t m, int64_t n, int64_t ncols, double* A, in
     double* T = ws + L.t_off + p * QR_NB * QR_NB;
     const int64_t mrows = m - c0;
     double* tau = ws + L.tau_off + p * QR_NB;
-    if (mrows <= 1024)
-      qr_panel_reg_kernel<1><<<1, 1024, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
-    else if (mrows <= 2048)
-      qr_panel_reg_kernel<2><<<1, 1024, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
-    else if (mrows <= 4096)
-      qr_panel_reg_kernel<4><<<1, 1024, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
+    constexpr int NT = QR_REG_THREADS;
+    if (mrows <= NT)
+      qr_panel_reg_kernel<1><<<1, NT, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
+    else if (mrows <= 2 * NT)
+      qr_panel_reg_kernel<2><<<1, NT, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
+    else if (mrows <= 4 * NT)
+      qr_panel_reg_kernel<4><<<1, NT, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
+    else if (mrows <= 8 * NT)
+      qr_panel_reg_kernel<8><<<1, NT, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
     else
       qr_panel_kernel<<<1, 1024, 0, st>>>(m, c0, nb, A, lda, V, T, tau, flag);
     PAMD_LAUNCH_CHECK();

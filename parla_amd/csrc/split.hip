@@ -125,14 +125,17 @@ __global__ __launch_bounds__(256) void sg_split_t_kernel(int64_t rows, int64_t c
 }
 
 // ws[ks][i][j] (FP32) = sum over p + q < P of L_p[i, K_ks] R_q[j, K_ks]^T.
-// K steps stream through two LDS buffers by direct global -> LDS loads
+// K steps stream through three LDS buffers by direct global -> LDS loads
 // (global_load_lds_dwordx4: a wavefront moves one 1 KiB block per
-// instruction, no registers involved): step s + 1 is loaded while step s is
-// multiplied.  Direct-to-LDS loads of one wavefront can complete out of issue
-// order, so a partial vmcnt does not say which of them are in: every step
-// starts with vmcnt(0) and a barrier.  The buffers are separate arrays and
-// the loop is unrolled by two, so every access names its buffer at compile
-// time.
+// instruction, no registers involved), two steps ahead.  Direct-to-LDS loads
+// of one wavefront can complete out of issue order, so only vmcnt(0) says
+// that a wavefront's loads are in.  The steps therefore belong alternately to
+// two groups of two wavefronts: the group that owns step s loads all of its
+// blocks, and at step s it waits with vmcnt(0) for them alone (its next
+// loads, for step s + 2, are issued after that wait), then the barrier makes
+// them visible to all.  Two steps stay in flight, and every wait is a full
+// one.  The buffers are separate arrays and the loop is unrolled by three,
+// so every access names its buffer at compile time.
 template <int P, int WM, int WN, int MB, int NB>
 __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t kc, int mtiles, int ntiles,
                                                              int nsplit, const uint16_t* __restrict__ L,
@@ -143,9 +146,13 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
   static_assert(TM == SG_TM && WM * WN == 4, "four wavefronts on a 128-row tile");
   constexpr int LB = TM / 16, RB = TN / 16;  // 1 KiB blocks per piece per K step
   constexpr int NBLK = P * (LB + RB);        // blocks per K step: L pieces, then R pieces
-  __shared__ uint4 S0[NBLK][64], S1[NBLK][64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int G = 2, NWG = 4 / G;          // loading groups, wavefronts per group
+  static_assert(LB % NWG == 0, "row blocks of L split evenly over a group");
+  __shared__ uint4 S0[NBLK][64], S1[NBLK][64], S2[NBLK][64];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: block choices are scalar
   const int wm = wave % WM, wn = wave / WM;
+  const int grp = wave / NWG, wl = wave % NWG;
   const int frag = blk_frag(lane);
   const int lr = lane & 15, lk = lane >> 4;
   const uint32_t total = (uint32_t)mtiles * ntiles * nsplit;
@@ -160,22 +167,23 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
   const uint4* Rg = reinterpret_cast<const uint4*>(R);
   const int64_t lps4 = lps >> 3, rps4 = rps >> 3;
 
-  // wavefront w loads blocks w, w + 4, ... of step `step` into buffer `dst`
+  // the calling wavefront (of the group owning `step`) loads rows wl, wl + NWG,
+  // ... of every piece's block column of the step into buffer `dst`
   auto issue = [&](int step, uint4 (*dst)[64]) {
     const int64_t kb = kb0 + step;
 #pragma unroll
-    for (int i = 0; i < (NBLK + 3) / 4; ++i) {
-      const int b = wave + 4 * i;
-      if (b < NBLK) {
-        const uint4* src;
-        if (b < P * LB) {
-          const int p = b / LB, rb = b % LB;
-          src = Lg + p * lps4 + (((int64_t)mt * LB + rb) * kbn + kb) * 64;
-        } else {
-          const int q = (b - P * LB) / RB, rb = (b - P * LB) % RB;
-          src = Rg + q * rps4 + (((int64_t)nt * RB + rb) * kbn + kb) * 64;
-        }
-        blk_load_lds(src, &dst[b][0], lane);
+    for (int p = 0; p < P; ++p) {
+#pragma unroll
+      for (int r = 0; r < LB / NWG; ++r) {
+        const int rb = wl + NWG * r;
+        blk_load_lds(Lg + p * lps4 + (((int64_t)mt * LB + rb) * kbn + kb) * 64, &dst[p * LB + rb][0], lane);
+      }
+#pragma unroll
+      for (int r = 0; r < (RB + NWG - 1) / NWG; ++r) {
+        // an odd RB: the last row is loaded by both wavefronts (same bytes, same place), which keeps
+        // every load unconditional
+        const int rb = (RB % NWG == 0) ? wl + NWG * r : (wl + NWG * r < RB ? wl + NWG * r : RB - 1);
+        blk_load_lds(Rg + p * rps4 + (((int64_t)nt * RB + rb) * kbn + kb) * 64, &dst[P * LB + p * RB + rb][0], lane);
       }
     }
   };
@@ -186,14 +194,14 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = sg_f4{0.f, 0.f, 0.f, 0.f};
 
-  // one K step: wait for every load of this wavefront (the step's blocks),
-  // barrier (every wavefront's blocks are in, and the other buffer, read in
-  // the previous step, is free), issue the next step's loads into it, then
-  // read the fragments and run the MFMAs
-  auto step = [&](int s, uint4 (*cur)[64], uint4 (*nxt)[64]) {
-    __builtin_amdgcn_s_waitcnt(0x3F70);  // vmcnt(0)
+  // one K step: the owning group waits for its loads (the step's blocks),
+  // barrier (every block is in, and the buffer read in the previous step is
+  // free), the owner issues the loads of step s + 2 into that buffer, then
+  // every wavefront reads its fragments and runs the MFMAs
+  auto step = [&](int s, uint4 (*cur)[64], uint4 (*ahead)[64]) {
+    const bool own = (s % G) == grp;
+    if (own) __builtin_amdgcn_s_waitcnt(0x3F70);  // vmcnt(0)
     asm volatile("s_barrier" ::: "memory");
-    if (s + 1 < nsteps) issue(s + 1, nxt);
     sg_bf16x8 a[P][MB], b[P][NB];
 #pragma unroll
     for (int p = 0; p < P; ++p)
@@ -205,6 +213,8 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb)
         b[q][nb] = *reinterpret_cast<const sg_bf16x8*>(&cur[P * LB + q * RB + wn * NB + nb][frag]);
+    asm volatile("" ::: "memory");
+    if (own && s + G < nsteps) issue(s + G, ahead);
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
@@ -218,10 +228,12 @@ __global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t
         }
   };
 
-  if (nsteps > 0) issue(0, S0);
-  for (int s = 0; s < nsteps; s += 2) {
-    step(s, S0, S1);
+  if (grp == 0 && nsteps > 0) issue(0, S0);
+  if (grp == 1 && nsteps > 1) issue(1, S1);
+  for (int s = 0; s < nsteps; s += 3) {
+    step(s, S0, S2);
     if (s + 1 < nsteps) step(s + 1, S1, S0);
+    if (s + 2 < nsteps) step(s + 2, S2, S1);
   }
   // D layout of v_mfma_f32_16x16x32_bf16: lane l, item r -> row 4 (l / 16) + r, column l % 16
   float* w = ws + (int64_t)ks * M * npad;
