@@ -192,6 +192,23 @@ int pamd_transpose_f64(int64_t rows, int64_t cols, const double* A, int64_t lda,
 int pamd_potrf_upper_f64(int64_t n, double* A, int64_t lda, int* info,
                          void* stream);
 
+/* out = A + alpha B + sigma I (upper_half 0) or H(A + alpha B + sigma I)
+ * (upper_half 1: the upper triangle with the diagonal halved, zeros below),
+ * k x k with k <= 512, B optional; nrm (optional, one device double) gets
+ * ||A + alpha B + sigma I||_F.  One launch for the small steps of the Newton
+ * second round of CholeskyQR2 inside the QB stabilizer
+ * (utils/linalg_wrappers.py:6-7). */
+int pamd_small_combine_f64(int64_t k, const double* A, int64_t lda, const double* B,
+                           int64_t ldb, double alpha, double sigma, int upper_half,
+                           double* out, int64_t ldo, double* nrm, void* stream);
+
+/* out[0..3] (device doubles) = min |G_ii|, max |G_ii|, enorm[0] (0 when
+ * enorm is NULL), info[0]: the CholeskyQR2 acceptance figures (pivot,
+ * conditioning estimate, size of the second-round correction) for a single
+ * read-back; 1 <= k <= 512. */
+int pamd_cholqr_stats_f64(int64_t k, const double* G, int64_t ldg, const int* info,
+                          const double* enorm, double* out, void* stream);
+
 /* Rinv (n x n, zeros below the diagonal) = R^{-1} for upper-triangular R
  * (n <= 2048); one triangular solve per column, all columns in parallel. */
 int pamd_trtri_upper_f64(int64_t n, const double* R, int64_t ldr, double* Rinv,

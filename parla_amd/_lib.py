@@ -63,6 +63,9 @@ SIGNATURES = {
     "pamd_lu_pl_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
     "pamd_potrf_upper_f64": (c_int, [c_i64, c_vp, c_i64, c_vp, c_vp]),
     "pamd_trtri_upper_f64": (c_int, [c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
+    "pamd_small_combine_f64": (c_int, [c_i64, c_vp, c_i64, c_vp, c_i64, c_dbl, c_dbl, c_int, c_vp, c_i64, c_vp,
+                                       c_vp]),
+    "pamd_cholqr_stats_f64": (c_int, [c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "pamd_trsm_mfma_workspace": (c_int, [c_i64, ctypes.POINTER(c_i64)]),
     "pamd_trsm_right_upper_mfma_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
     "pamd_trsm_right_upper_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),

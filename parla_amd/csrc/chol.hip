@@ -521,6 +521,81 @@ __global__ __launch_bounds__(1024) void trsm_right_kernel(int64_t m, int64_t n, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Small k x k helpers of CholeskyQR2, one workgroup each (k <= 512):
+//   combine: out = A + alpha B + sigma I, or H(A + alpha B + sigma I) where H
+//            keeps the upper triangle and halves the diagonal; optionally
+//            ||A + alpha B + sigma I||_F (rows summed in order by each thread,
+//            then the waves in order);
+//   stats:   min |G_ii|, max |G_ii|, the norm above and the potrf info, packed
+//            into four doubles for one read-back.
+// ---------------------------------------------------------------------------
+constexpr int SC_THREADS = 1024;
+
+__global__ __launch_bounds__(SC_THREADS) void small_combine_kernel(int k, const double* __restrict__ A, int64_t lda,
+                                                                  const double* __restrict__ B, int64_t ldb,
+                                                                  double alpha, double sigma, int upper_half,
+                                                                  double* __restrict__ out, int64_t ldo,
+                                                                  double* __restrict__ nrm) {
+  __shared__ double red[SC_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double ss = 0.0;
+  const int64_t total = (int64_t)k * k;
+  for (int64_t e = tid; e < total; e += SC_THREADS) {
+    const int i = (int)(e / k), j = (int)(e % k);
+    double v = A[i * lda + j];
+    if (B != nullptr) v += alpha * B[i * ldb + j];
+    if (i == j) v += sigma;
+    ss += v * v;
+    if (upper_half) v = (j > i) ? v : ((j == i) ? 0.5 * v : 0.0);
+    out[i * ldo + j] = v;
+  }
+  if (nrm != nullptr) {
+    ss = wave_sum(ss);
+    if (lane == 0) red[wave] = ss;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int w = 0; w < SC_THREADS / 64; ++w) t += red[w];
+      nrm[0] = sqrt(t);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void cholqr_stats_kernel(int k, const double* __restrict__ G, int64_t ldg,
+                                                          const int* __restrict__ info,
+                                                          const double* __restrict__ enorm,
+                                                          double* __restrict__ out) {
+  __shared__ double smin[4], smax[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double mn = INFINITY, mx = 0.0;
+  for (int i = tid; i < k; i += 256) {
+    const double d = fabs(G[(int64_t)i * ldg + i]);
+    mn = fmin(mn, d);
+    mx = fmax(mx, d);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    mn = fmin(mn, __shfl_xor(mn, off, 64));
+    mx = fmax(mx, __shfl_xor(mx, off, 64));
+  }
+  if (lane == 0) {
+    smin[wave] = mn;
+    smax[wave] = mx;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < 4; ++w) {
+      mn = fmin(mn, smin[w]);
+      mx = fmax(mx, smax[w]);
+    }
+    out[0] = mn;
+    out[1] = mx;
+    out[2] = (enorm != nullptr) ? enorm[0] : 0.0;
+    out[3] = (double)info[0];
+  }
+}
+
 }  // namespace
 
 extern "C" int pamd_potrf_upper_f64(int64_t n, double* A, int64_t lda, int* info, void* stream) {
@@ -625,6 +700,35 @@ extern "C" int pamd_trsm_right_upper_mfma_f64(int64_t m, int64_t k, const double
   PAMD_LAUNCH_CHECK();
   trsm_strip_kernel<<<(unsigned)pamd_cdiv(m, TS_B), 256, trsm_strip_lds((int)k), st>>>(m, (int)k, Y, ldy, R, ldr,
                                                                      (const double*)workspace, X, ldx);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_small_combine_f64(int64_t k, const double* A, int64_t lda, const double* B, int64_t ldb,
+                                      double alpha, double sigma, int upper_half, double* out, int64_t ldo,
+                                      double* nrm, void* stream) {
+  PAMD_ARG_CHECK(k >= 0 && k <= 512, 1);
+  PAMD_ARG_CHECK(A != nullptr || k == 0, 2);
+  PAMD_ARG_CHECK(lda >= k && lda >= 1, 3);
+  PAMD_ARG_CHECK(B == nullptr || ldb >= k, 5);
+  PAMD_ARG_CHECK(upper_half == 0 || upper_half == 1, 8);
+  PAMD_ARG_CHECK(out != nullptr || k == 0, 9);
+  PAMD_ARG_CHECK(ldo >= k && ldo >= 1, 10);
+  if (k == 0 && nrm == nullptr) return PAMD_OK;
+  small_combine_kernel<<<1, SC_THREADS, 0, (hipStream_t)stream>>>((int)k, A, lda, B, ldb, alpha, sigma, upper_half,
+                                                                 out, ldo, nrm);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_cholqr_stats_f64(int64_t k, const double* G, int64_t ldg, const int* info, const double* enorm,
+                                     double* out, void* stream) {
+  PAMD_ARG_CHECK(k >= 1 && k <= 512, 1);
+  PAMD_ARG_CHECK(G != nullptr, 2);
+  PAMD_ARG_CHECK(ldg >= k, 3);
+  PAMD_ARG_CHECK(info != nullptr, 4);
+  PAMD_ARG_CHECK(out != nullptr, 6);
+  cholqr_stats_kernel<<<1, 256, 0, (hipStream_t)stream>>>((int)k, G, ldg, info, enorm, out);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

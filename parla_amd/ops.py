@@ -298,28 +298,33 @@ def _chol_round(Q, info):
     return G, trsm_mfma(Q, G)  # Q R^{-1}
 
 
+def small_combine(A, B=None, alpha=0.0, sigma=0.0, upper_half=False, out=None, nrm=None):
+    """out = A + alpha B + sigma I, or its upper triangle with the diagonal
+    halved (upper_half); nrm (a one-element device tensor) receives
+    ||A + alpha B + sigma I||_F.  One launch (k <= 512)."""
+    k = A.shape[0]
+    if out is None:
+        out = torch.empty((k, k), dtype=F64, device=A.device)
+    check(lib().pamd_small_combine_f64(k, _p(A), A.stride(0), _p(B), B.stride(0) if B is not None else k,
+                                       float(alpha), float(sigma), 1 if upper_half else 0, _p(out), out.stride(0),
+                                       _p(nrm), stream(A.device)), "pamd_small_combine_f64")
+    return out
+
+
 def _newton_round(Q):
     """Second CholeskyQR round for a nearly orthonormal Q (Q^T Q = I + E):
     the Cholesky factor of I + E is I + X with X + X^T = E - X^T X, solved by
     two fixed-point steps X = H(E - X^T X) (H takes the upper triangle and
     halves the diagonal; the error after them is O(||E||^3)), and
     (I + X)^{-1} = I - X + X^2 + O(||X||^3).  Three k x k products and one
-    m x k x k GEMM instead of potrf and trtri.  Returns (Q (I + X)^{-1}, ||E||_F)."""
-    k = Q.shape[1]
-    E = gemm(Q.t(), Q)
-    E.diagonal().sub_(1.0)
-
-    def H(F):
-        Xh = torch.triu(F)
-        Xh.diagonal().mul_(0.5)
-        return Xh
-
-    X = H(E)
-    X = H(E - gemm(X.t(), X))
-    M = gemm(X, X)
-    M -= X
-    M.diagonal().add_(1.0)
-    return gemm(Q, M), torch.linalg.matrix_norm(E)
+    m x k x k GEMM instead of potrf and trtri.  Returns (Q (I + X)^{-1}, ||E||_F)
+    with ||E||_F as a one-element device tensor."""
+    QtQ = gemm(Q.t(), Q)
+    enorm = torch.empty(1, dtype=F64, device=Q.device)
+    X = small_combine(QtQ, sigma=-1.0, upper_half=True, nrm=enorm)             # H(E)
+    X = small_combine(QtQ, gemm(X.t(), X), alpha=-1.0, sigma=-1.0, upper_half=True)  # H(E - X^T X)
+    M = small_combine(gemm(X, X), X, alpha=-1.0, sigma=1.0)                     # I - X + X^2
+    return gemm(Q, M), enorm
 
 
 def cholqr2(Y, rounds=2):
@@ -334,16 +339,17 @@ def cholqr2(Y, rounds=2):
     m, k = Y.shape
     if k == 0 or k > CHOLQR_MAX_K or m < k:
         return None
-    info = torch.zeros(2, dtype=torch.int32, device=Y.device)
+    info = torch.empty(2, dtype=torch.int32, device=Y.device)  # potrf writes info[0] (0 or the failing pivot)
     G, Q1 = _chol_round(Y, info[0:1])
-    diag = torch.diagonal(G)
     if rounds == 1:
-        Q, enorm = Q1, torch.zeros((), dtype=F64, device=Y.device)
+        Q, enorm = Q1, None
     else:
         Q, enorm = _newton_round(Q1)
     # one synchronisation: pivot, conditioning estimate, size of E
-    dd = torch.cat([diag.abs().min().reshape(1), diag.abs().max().reshape(1), enorm.reshape(1),
-                    info[0:1].to(F64)]).cpu().numpy()
+    stats = torch.empty(4, dtype=F64, device=Y.device)
+    check(lib().pamd_cholqr_stats_f64(k, _p(G), G.stride(0), _p(info), _p(enorm), _p(stats), stream(Y.device)),
+          "pamd_cholqr_stats_f64")
+    dd = stats.cpu().numpy()
     if dd[3] != 0 or not np.all(np.isfinite(dd)) or dd[0] <= 0 or dd[1] / dd[0] > CHOLQR_MAX_COND:
         return None
     if rounds == 2 and dd[2] > NEWTON_MAX_E:

@@ -322,6 +322,33 @@ def test_cholqr2_declines_rank_deficient(ops):
     assert la.norm(Q.T @ Q - np.eye(20)) <= 1e-12
 
 
+@pytest.mark.parametrize("k", [1, 7, 266, 512])
+def test_small_combine_and_cholqr_stats(ops, k):
+    """The CholeskyQR2 helpers: A + alpha B + sigma I and its halved upper
+    triangle, the Frobenius norm, and the packed acceptance figures."""
+    import torch
+    rng = np.random.default_rng(k)
+    A = rng.standard_normal((k, k))
+    B = rng.standard_normal((k, k))
+    full = A - 0.5 * B + 2.0 * np.eye(k)
+    nrm = torch.empty(1, dtype=torch.float64, device="cuda")
+    got = host(ops.small_combine(dev(A), dev(B), alpha=-0.5, sigma=2.0, nrm=nrm))
+    np.testing.assert_allclose(got, full, rtol=0, atol=1e-15 * np.abs(full).max())
+    assert abs(nrm.item() - la.norm(full)) <= 1e-13 * la.norm(full)
+    h = host(ops.small_combine(dev(A), sigma=-1.0, upper_half=True))
+    E = A - np.eye(k)
+    want = np.triu(E)
+    want[np.diag_indices(k)] *= 0.5
+    np.testing.assert_array_equal(h, want)
+    G = dev(np.diag(rng.uniform(0.5, 3.0, k)) * np.sign(rng.standard_normal(k))[:, None])
+    info = torch.tensor([3], dtype=torch.int32, device="cuda")
+    stats = torch.empty(4, dtype=torch.float64, device="cuda")
+    ops.check(ops.lib().pamd_cholqr_stats_f64(k, ops._p(G), G.stride(0), ops._p(info), ops._p(nrm),
+                                              ops._p(stats), ops.stream()), "stats")
+    d = np.abs(np.diag(host(G)))
+    assert host(stats).tolist() == [d.min(), d.max(), nrm.item(), 3.0]
+
+
 @pytest.mark.parametrize("n", [1, 7, 64, 266, 1024])
 def test_trtri_matches_scipy(ops, n):
     rng = np.random.default_rng(n)
