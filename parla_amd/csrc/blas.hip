@@ -1,7 +1,8 @@
 // Dense FP64 kernels: MFMA GEMM, GEMV, 2-norm, upper-triangular solves.
 //
-// GEMM: 64x64 output tile per 256-thread workgroup (four 64-wide wavefronts,
-// each owning a 32x32 quadrant = 2x2 tiles of v_mfma_f64_16x16x4f64), K staged
+// GEMM: 64 or 96 x 64 or 96 output tile per 256-thread workgroup (four
+// wavefronts, each owning a quadrant of 16x16 v_mfma_f64_16x16x4f64 blocks;
+// the tile edge per dimension is the one that pads that dimension less), K staged
 // through LDS 16 at a time with the next K tile prefetched into registers
 // while the current one is multiplied.  Optional split-K writes per-slice
 // partials that a second kernel sums in slice order (deterministic).
@@ -12,87 +13,103 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int GBM = 64, GBN = 64, GBK = 16;
+constexpr int GBK = 16;
 
-// op(A)[m][k] for the tile (m0.., k0..) -> registers (4 values per thread)
-template <int TA>
+// op(A)[m][k] for the BM x 16 tile (m0.., k0..) -> registers, two consecutive
+// doubles per slot, BM / 32 slots per thread
+template <int TA, int BM>
 __device__ __forceinline__ void load_a(const double* __restrict__ A, int64_t lda, int64_t M, int64_t K,
-                                       int64_t m0, int64_t k0, double (&r)[4]) {
+                                       int64_t m0, int64_t k0, double (&r)[BM / 16]) {
   const int t = threadIdx.x;
-  if (TA == 0) {  // A is M x K: rows of 16 contiguous k
-    const int64_t m = m0 + t / 4;
-    const int64_t kq = k0 + (t % 4) * 4;
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
-      r[e] = (m < M && kq + e < K) ? A[m * lda + kq + e] : 0.0;
-  } else {  // A is K x M: rows of 64 contiguous m
-    const int64_t k = k0 + t / 16;
-    const int64_t mq = m0 + (t % 16) * 4;
+  for (int i = 0; i < BM / 32; ++i) {
+    const int sl = t + 256 * i;
+    if (TA == 0) {  // A is M x K: rows of 16 contiguous k, 8 slots per row
+      const int64_t m = m0 + sl / 8;
+      const int64_t kq = k0 + (sl % 8) * 2;
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
-      r[e] = (k < K && mq + e < M) ? A[k * lda + mq + e] : 0.0;
+      for (int e = 0; e < 2; ++e) r[2 * i + e] = (m < M && kq + e < K) ? A[m * lda + kq + e] : 0.0;
+    } else {  // A is K x M: rows of BM contiguous m
+      const int64_t k = k0 + sl / (BM / 2);
+      const int64_t mq = m0 + (sl % (BM / 2)) * 2;
+#pragma unroll
+      for (int e = 0; e < 2; ++e) r[2 * i + e] = (k < K && mq + e < M) ? A[k * lda + mq + e] : 0.0;
+    }
   }
 }
 
-template <int TA>
-__device__ __forceinline__ void store_a(double (*As)[GBM + 1], const double (&r)[4]) {
+template <int TA, int BM>
+__device__ __forceinline__ void store_a(double (*As)[BM + 1], const double (&r)[BM / 16]) {
   const int t = threadIdx.x;
-  if (TA == 0) {
-    const int m = t / 4, kq = (t % 4) * 4;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) As[kq + e][m] = r[e];
-  } else {
-    const int k = t / 16, mq = (t % 16) * 4;
+  for (int i = 0; i < BM / 32; ++i) {
+    const int sl = t + 256 * i;
+    if (TA == 0) {
+      const int m = sl / 8, kq = (sl % 8) * 2;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) As[k][mq + e] = r[e];
+      for (int e = 0; e < 2; ++e) As[kq + e][m] = r[2 * i + e];
+    } else {
+      const int k = sl / (BM / 2), mq = (sl % (BM / 2)) * 2;
+#pragma unroll
+      for (int e = 0; e < 2; ++e) As[k][mq + e] = r[2 * i + e];
+    }
   }
 }
 
-// op(B)[k][n] for the tile (k0.., n0..)
-template <int TB>
+// op(B)[k][n] for the 16 x BN tile (k0.., n0..)
+template <int TB, int BN>
 __device__ __forceinline__ void load_b(const double* __restrict__ B, int64_t ldb, int64_t N, int64_t K,
-                                       int64_t n0, int64_t k0, double (&r)[4]) {
+                                       int64_t n0, int64_t k0, double (&r)[BN / 16]) {
   const int t = threadIdx.x;
-  if (TB == 0) {  // B is K x N
-    const int64_t k = k0 + t / 16;
-    const int64_t nq = n0 + (t % 16) * 4;
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
-      r[e] = (k < K && nq + e < N) ? B[k * ldb + nq + e] : 0.0;
-  } else {  // B is N x K
-    const int64_t n = n0 + t / 4;
-    const int64_t kq = k0 + (t % 4) * 4;
+  for (int i = 0; i < BN / 32; ++i) {
+    const int sl = t + 256 * i;
+    if (TB == 0) {  // B is K x N: rows of BN contiguous n
+      const int64_t k = k0 + sl / (BN / 2);
+      const int64_t nq = n0 + (sl % (BN / 2)) * 2;
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
-      r[e] = (n < N && kq + e < K) ? B[n * ldb + kq + e] : 0.0;
+      for (int e = 0; e < 2; ++e) r[2 * i + e] = (k < K && nq + e < N) ? B[k * ldb + nq + e] : 0.0;
+    } else {  // B is N x K: rows of 16 contiguous k
+      const int64_t n = n0 + sl / 8;
+      const int64_t kq = k0 + (sl % 8) * 2;
+#pragma unroll
+      for (int e = 0; e < 2; ++e) r[2 * i + e] = (n < N && kq + e < K) ? B[n * ldb + kq + e] : 0.0;
+    }
   }
 }
 
-template <int TB>
-__device__ __forceinline__ void store_b(double (*Bs)[GBN + 1], const double (&r)[4]) {
+template <int TB, int BN>
+__device__ __forceinline__ void store_b(double (*Bs)[BN + 1], const double (&r)[BN / 16]) {
   const int t = threadIdx.x;
-  if (TB == 0) {
-    const int k = t / 16, nq = (t % 16) * 4;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) Bs[k][nq + e] = r[e];
-  } else {
-    const int n = t / 4, kq = (t % 4) * 4;
+  for (int i = 0; i < BN / 32; ++i) {
+    const int sl = t + 256 * i;
+    if (TB == 0) {
+      const int k = sl / (BN / 2), nq = (sl % (BN / 2)) * 2;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) Bs[kq + e][n] = r[e];
+      for (int e = 0; e < 2; ++e) Bs[k][nq + e] = r[2 * i + e];
+    } else {
+      const int n = sl / 8, kq = (sl % 8) * 2;
+#pragma unroll
+      for (int e = 0; e < 2; ++e) Bs[kq + e][n] = r[2 * i + e];
+    }
   }
 }
 
-// grid: (ceil(N/64), ceil(M/64), splits).  Slice z covers K range
+// BM x BN output tile per workgroup (BM, BN in {64, 96}); four wavefronts in
+// 2 x 2, each owning a (BM/2) x (BN/2) quadrant of 16 x 16 MFMA blocks.
+// grid: (ceil(N/BN), ceil(M/BM), splits).  Slice z covers K range
 // [z*kchunk, min(K, (z+1)*kchunk)).  With splits > 1 the raw product of each
 // slice goes to part[z] (M x N, ld N); otherwise C = alpha*prod + beta*C.
-template <int TA, int TB>
+template <int TA, int TB, int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk,
                                                        double alpha, const double* __restrict__ A, int64_t lda,
                                                        const double* __restrict__ B, int64_t ldb, double beta,
                                                        double* __restrict__ C, int64_t ldc,
                                                        double* __restrict__ part) {
-  __shared__ double As[GBK][GBM + 1];
-  __shared__ double Bs[GBK][GBN + 1];
+  constexpr int IM = BM / 32, JN = BN / 32;  // MFMA blocks per wavefront
+  __shared__ double As[GBK][BM + 1];
+  __shared__ double Bs[GBK][BN + 1];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -114,54 +131,55 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
     ty = t2 % gy;
     tx = t2 / gy;
   }
-  const int64_t m0 = (int64_t)ty * GBM;
-  const int64_t n0 = (int64_t)tx * GBN;
+  const int64_t m0 = (int64_t)ty * BM;
+  const int64_t n0 = (int64_t)tx * BN;
   const int64_t kbeg = (int64_t)tz * kchunk;
   const int64_t kend = min(K, kbeg + kchunk);
 
-  d4 acc[2][2];
+  d4 acc[IM][JN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < IM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < JN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  double ra[4], rb[4];
+  double ra[BM / 16], rb[BN / 16];
   if (kbeg < kend) {
-    load_a<TA>(A, lda, M, kend, m0, kbeg, ra);
-    load_b<TB>(B, ldb, N, kend, n0, kbeg, rb);
+    load_a<TA, BM>(A, lda, M, kend, m0, kbeg, ra);
+    load_b<TB, BN>(B, ldb, N, kend, n0, kbeg, rb);
   }
   for (int64_t k0 = kbeg; k0 < kend; k0 += GBK) {
-    store_a<TA>(As, ra);
-    store_b<TB>(Bs, rb);
+    store_a<TA, BM>(As, ra);
+    store_b<TB, BN>(Bs, rb);
     __syncthreads();
     if (k0 + GBK < kend) {  // prefetch the next K tile while this one is used
-      load_a<TA>(A, lda, M, kend, m0, k0 + GBK, ra);
-      load_b<TB>(B, ldb, N, kend, n0, k0 + GBK, rb);
+      load_a<TA, BM>(A, lda, M, kend, m0, k0 + GBK, ra);
+      load_b<TB, BN>(B, ldb, N, kend, n0, k0 + GBK, rb);
     }
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 4) {
       const int kr = kk + (lane >> 4);
-      double a0 = As[kr][wm * 32 + (lane & 15)];
-      double a1 = As[kr][wm * 32 + 16 + (lane & 15)];
-      double b0 = Bs[kr][wn * 32 + (lane & 15)];
-      double b1 = Bs[kr][wn * 32 + 16 + (lane & 15)];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      double a[IM], b[JN];
+#pragma unroll
+      for (int i = 0; i < IM; ++i) a[i] = As[kr][wm * (BM / 2) + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < JN; ++j) b[j] = Bs[kr][wn * (BN / 2) + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < IM; ++i)
+#pragma unroll
+        for (int j = 0; j < JN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
 
   // D layout of v_mfma_f64_16x16x4f64: lane l, item r -> row l/16 + 4*r, col l%16
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < IM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < JN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
-        const int64_t col = n0 + wn * 32 + j * 16 + (lane & 15);
+        const int64_t row = m0 + wm * (BM / 2) + i * 16 + (lane >> 4) + 4 * r;
+        const int64_t col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
         if (row < M && col < N) {
           const double v = acc[i][j][r];
           if (part) {
@@ -172,6 +190,13 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
           }
         }
       }
+}
+
+// Tile edge for a dimension of size d: 96 when that pads at least 5% less
+// than 64 (the power iteration's k = 266 pads to 288 instead of 320).
+inline int gemm_edge(int64_t d) {
+  const int64_t p64 = pamd_cdiv(d, 64) * 64, p96 = pamd_cdiv(d, 96) * 96;
+  return (p96 * 20 <= p64 * 19) ? 96 : 64;
 }
 
 // C = alpha * sum_z part[z] + beta * C, summed in slice order.
@@ -187,13 +212,18 @@ __global__ void splitk_reduce_kernel(int64_t M, int64_t N, int splits, double al
 }
 
 int gemm_splits(int64_t M, int64_t N, int64_t K) {
-  const int64_t tiles = pamd_cdiv(M, GBM) * pamd_cdiv(N, GBN);
+  const int64_t tiles = pamd_cdiv(M, gemm_edge(M)) * pamd_cdiv(N, gemm_edge(N));
   if (K < 4 * 256) return 1;
-  // a large output with a long K (the power iteration's A S, 640 tiles): split
-  // K until there are 2048 workgroups, eight per CU, so enough wavefronts are
-  // resident to hide MFMA and load latency; the few FP64 partials are cheap
-  // next to the K reduction
-  if (tiles >= 512) return (tiles >= 2048 || K < 2048) ? 1 : (int)pamd_cdiv(2048, tiles);
+  // a large output with a long K (the power iteration's A S, A^T Y and Q^T A
+  // at 8192 x 4096, k = 266): split K until there are about 3072 workgroups,
+  // twelve per CU, so enough wavefronts are resident to hide MFMA and load
+  // latency (measured best of 2048 / 3072 / 4096); the FP64 partials are
+  // cheap next to the K reduction
+  if (M * N >= (1LL << 20) && K >= 2048) {
+    if (tiles >= 3072) return 1;
+    const int64_t s = imin64(pamd_cdiv(3072, tiles), K / 256);
+    return (int)imax64(1, imin64(s, 64));
+  }
   // small outputs: about 1024 workgroups, at least 256 K per slice
   const int64_t target = 1024;
   int64_t s = pamd_cdiv(target, tiles);
@@ -332,15 +362,23 @@ extern "C" int pamd_gemm_f64(int transA, int transB, int64_t M, int64_t N, int64
   const int64_t kchunk = splits > 1 ? pamd_cdiv(pamd_cdiv(K, splits), GBK) * GBK : imax64(K, 1);
   if (splits > 1) splits = (int)pamd_cdiv(K, kchunk);
   double* part = splits > 1 ? (double*)workspace : nullptr;
-  dim3 grid((unsigned)pamd_cdiv(N, GBN), (unsigned)pamd_cdiv(M, GBM), (unsigned)splits);
-  if (transA == 0 && transB == 0)
-    gemm_f64_kernel<0, 0><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part);
-  else if (transA == 0 && transB == 1)
-    gemm_f64_kernel<0, 1><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part);
-  else if (transA == 1 && transB == 0)
-    gemm_f64_kernel<1, 0><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part);
-  else
-    gemm_f64_kernel<1, 1><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part);
+  const int bm = gemm_edge(M), bn = gemm_edge(N);
+  dim3 grid((unsigned)pamd_cdiv(N, bn), (unsigned)pamd_cdiv(M, bm), (unsigned)splits);
+#define PAMD_GEMM(TA, TB, BM, BN) \
+  gemm_f64_kernel<TA, TB, BM, BN><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part)
+#define PAMD_GEMM_T(BM, BN)                                  \
+  do {                                                       \
+    if (transA == 0 && transB == 0) PAMD_GEMM(0, 0, BM, BN); \
+    else if (transA == 0) PAMD_GEMM(0, 1, BM, BN);           \
+    else if (transB == 0) PAMD_GEMM(1, 0, BM, BN);           \
+    else PAMD_GEMM(1, 1, BM, BN);                            \
+  } while (0)
+  if (bm == 64 && bn == 64) PAMD_GEMM_T(64, 64);
+  else if (bm == 64) PAMD_GEMM_T(64, 96);
+  else if (bn == 64) PAMD_GEMM_T(96, 64);
+  else PAMD_GEMM_T(96, 96);
+#undef PAMD_GEMM_T
+#undef PAMD_GEMM
   PAMD_LAUNCH_CHECK();
   if (splits > 1) {
     const int64_t tot = M * N;
