@@ -52,7 +52,36 @@ __device__ __forceinline__ uint16_t bf16_bits_of_int(double a) {
 // times 32, rounded to the nearest integer, is this operator
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint16_t gauss_q_value(double z) {
-  return bf16_bits_of_int(fmin(256.0, fmax(-256.0, rint(32.0 * z))));
+  return bf16_bits_of_int(fmin(256.0, fmax(-256.0, rint(32.0 * z))) + 0.0);  // + 0.0: zero is +0
+}
+
+// The quantised values of normal_pair(seed, pair): the pair is first formed
+// in FP32 from the same uniforms.  Its error is below 3e-4 in units of
+// 1/32: the uniforms rounded to FP32 (2^-24 relative), logf / sincospif
+// within 2 ulp, sqrtf and the products within 1 ulp give |z32 - z| <= 4e-6
+// + 6e-7 |z| where 32 z can reach a rounding boundary (|z| >= 1/64).  When
+// either 32 z32 lies within 1e-3 of a half-integer (about one pair in 250)
+// the pair is recomputed in FP64, so the result is always that of
+// gauss_q_value on the FP64 pair.
+__device__ __forceinline__ void normal_pair_q(uint64_t seed, uint64_t pair, uint16_t& q0, uint16_t& q1) {
+  const U4 r = philox4x32_10(U4{(uint32_t)pair, (uint32_t)(pair >> 32), 0x6A09E667u, 0u}, (uint32_t)seed,
+                             (uint32_t)(seed >> 32));
+  const float u1 = (float)u53_open0(r.x, r.y);
+  const float u2 = (float)u53_open0(r.z, r.w);
+  const float rad = sqrtf(-2.0f * logf(u1));
+  float sf, cf;
+  sincospif(2.0f * u2, &sf, &cf);
+  const float t0 = 32.0f * (rad * cf), t1 = 32.0f * (rad * sf);
+  const float d0 = fabsf(t0 - (floorf(t0) + 0.5f)), d1 = fabsf(t1 - (floorf(t1) + 0.5f));
+  if (d0 < 1e-3f || d1 < 1e-3f) {
+    double z0, z1;
+    normal_pair(seed, pair, z0, z1);
+    q0 = gauss_q_value(z0);
+    q1 = gauss_q_value(z1);
+  } else {
+    q0 = bf16_bits_of_int((double)fminf(256.0f, fmaxf(-256.0f, rintf(t0))) + 0.0);
+    q1 = bf16_bits_of_int((double)fminf(256.0f, fmaxf(-256.0f, rintf(t1))) + 0.0);
+  }
 }
 
 // Blocked Q (rows_out x kp, the layout of blocked.h; zeros for i >= rows or
@@ -67,7 +96,7 @@ __global__ __launch_bounds__(256) void gauss_q_kernel(int64_t rows, int64_t m, i
   const int64_t k0 = (blk % kbn) * 32 + 8 * c;
   uint16_t v[8];
   uint64_t pair = ~0ull;
-  double z0 = 0.0, z1 = 0.0;
+  uint16_t q0 = 0, q1 = 0;
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
     v[t] = 0;
@@ -76,9 +105,9 @@ __global__ __launch_bounds__(256) void gauss_q_kernel(int64_t rows, int64_t m, i
       const uint64_t e = offset + (uint64_t)(i * m + k);
       if ((e >> 1) != pair) {
         pair = e >> 1;
-        normal_pair(seed, pair, z0, z1);
+        normal_pair_q(seed, pair, q0, q1);
       }
-      v[t] = gauss_q_value((e & 1) ? z1 : z0);
+      v[t] = (e & 1) ? q1 : q0;
     }
   }
   *reinterpret_cast<uint4*>(Q + blk * 512 + r * 32 + ((c ^ (r >> 2)) << 3)) = blk_pack(v);
@@ -105,8 +134,13 @@ __global__ __launch_bounds__(256) void oz_split_kernel(int64_t m, int64_t n, con
   const int64_t c = blockIdx.x;
   const int64_t j = (int64_t)blockIdx.y * 32 + jc;
   const int64_t k0 = c * OZ_CHUNK + rg * 32;
+  double x[32];  // this thread's 32 rows of column j, read once
   double mx = 0.0;
-  for (int r = 0; r < 32; ++r) mx = fmax(mx, fabs(oz_load(k0 + r, j, m, n, A, lda, b)));
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    x[r] = oz_load(k0 + r, j, m, n, A, lda, b);
+    mx = fmax(mx, fabs(x[r]));
+  }
   smax[rg][jc] = mx;
   __syncthreads();
   if (rg == 0) {
@@ -119,27 +153,33 @@ __global__ __launch_bounds__(256) void oz_split_kernel(int64_t m, int64_t n, con
   }
   __syncthreads();
   const int e = sexp[jc];
-  for (int p = 0; p < P; ++p) {
-    // pieces p of rows k0 .. k0+31, column j, in 4 stores of 8 BF16
-    uint16_t* dst = pieces + p * pstride;
-    for (int q = 0; q < 4; ++q) {
-      uint16_t h[8];
+  // rows k0 + 8q .. k0 + 8q + 7: all P pieces in one pass over the values,
+  // piece p = rint of y after p steps of y <- (y - rint(y)) * 512
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        double y = ldexp(oz_load(k0 + 8 * q + i, j, m, n, A, lda, b), 8 - e);
-        double a = 0.0;
-        for (int pp = 0; pp <= p; ++pp) {  // piece p of y
-          a = rint(y);
+  for (int q = 0; q < 4; ++q) {
+    uint16_t h[OZ_MAX_PIECES][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      double y = ldexp(x[8 * q + i], 8 - e);
+#pragma unroll
+      for (int p = 0; p < OZ_MAX_PIECES; ++p) {
+        if (p < P) {
+          const double a = rint(y);
+          h[p][i] = bf16_bits_of_int(a);
           y = (y - a) * 512.0;
         }
-        h[i] = bf16_bits_of_int(a);
       }
-      uint4 w;
-      w.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
-      w.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
-      w.z = (uint32_t)h[4] | ((uint32_t)h[5] << 16);
-      w.w = (uint32_t)h[6] | ((uint32_t)h[7] << 16);
-      *reinterpret_cast<uint4*>(dst + blk_off(j, k0 + 8 * q, ldk)) = w;
+    }
+#pragma unroll
+    for (int p = 0; p < OZ_MAX_PIECES; ++p) {
+      if (p < P) {
+        uint4 w;
+        w.x = (uint32_t)h[p][0] | ((uint32_t)h[p][1] << 16);
+        w.y = (uint32_t)h[p][2] | ((uint32_t)h[p][3] << 16);
+        w.z = (uint32_t)h[p][4] | ((uint32_t)h[p][5] << 16);
+        w.w = (uint32_t)h[p][6] | ((uint32_t)h[p][7] << 16);
+        *reinterpret_cast<uint4*>(pieces + p * pstride + blk_off(j, k0 + 8 * q, ldk)) = w;
+      }
     }
   }
 }

@@ -486,7 +486,7 @@ def quantize_q(Z, m=None):
     Z = as_device(Z)
     rows, mm = Z.shape
     kp = _round_up(max(mm, 1), OZ_CHUNK)
-    q = torch.clamp(torch.round(32.0 * Z), -256.0, 256.0).to(torch.bfloat16).view(torch.uint16)
+    q = (torch.clamp(torch.round(32.0 * Z), -256.0, 256.0) + 0.0).to(torch.bfloat16).view(torch.uint16)  # zero is +0
     return block_rows(q, _round_up(max(rows, 1), 128), kp)
 
 

@@ -178,3 +178,16 @@ def test_ozaki_sketch_full_size_long_groups(ops):
     bound = scale * Qf.abs().sum(dim=1, keepdim=True) * Ab.abs().max()
     bad = int(((out1 - full).abs() > 2.0 ** (-9 * P) * bound + 1e-13 * full.abs().max()).sum().item())
     assert bad == 0, bad
+
+
+def test_gauss_q_full_size_matches_quantised_fill(ops):
+    """The FP32 draw with its FP64 fallback gives, bit for bit, the quantised
+    FP64 fill at the 4096 x 262144 sketch size (zero as +0 on both sides)."""
+    rows, m, seed = 4096, 262144, 12345
+    _say(f"gauss_q full size {rows}x{m}")
+    dev = torch.device("cuda")
+    Q = ops.gauss_q(rows, m, seed, dev)
+    Z = ops.gaussian_fill(rows, m, 1.0, seed, dev)
+    Qf = ops.quantize_q(Z)
+    del Z
+    assert torch.equal(Q, Qf)
