@@ -131,7 +131,8 @@ def test_trsv_ignores_strict_lower_part(ops):
 # Householder QR, LU
 # --------------------------------------------------------------------------
 
-@pytest.mark.parametrize("m,n", [(1, 1), (10, 3), (64, 64), (800, 200), (300, 9), (2000, 266), (4100, 40)])
+@pytest.mark.parametrize("m,n", [(1, 1), (10, 3), (64, 64), (800, 200), (300, 9), (2000, 266), (4100, 40), (4096, 97),
+                                 (513, 200), (9000, 20)])
 def test_geqrf_orgqr_match_lapack(ops, m, n):
     _say(f"geqrf m={m} n={n}")
     rng = np.random.default_rng(m * 3 + n)
