@@ -693,7 +693,8 @@ def spo_mixed(A, b, tol, rng, sampling_factor=4, npieces=4, inner_tol=1e-6, max_
     of [A | b], Householder QR, B = A R^{-1} stored in FP32, then FP64
     iterative refinement x <- x + R^{-1} CG(B^T B, R^{-T} A^T r(x)) with the
     residuals in extended precision, until eta(x) <= tol or a step no longer
-    halves eta.  Returns the x with the smallest eta and the eta history."""
+    halves eta.  Each CG solve stops at the relative residual
+    min(0.1, max(inner_tol, 0.1 tol / eta)).  Returns the x with the smallest eta and the eta history."""
     A = np.asarray(A, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     m, n = A.shape
@@ -721,6 +722,9 @@ def spo_mixed(A, b, tol, rng, sampling_factor=4, npieces=4, inner_tol=1e-6, max_
         if eta <= tol or (step > 0 and eta > 0.5 * etas[-2]) or step == max_refine:
             break
         rhs = la.solve_triangular(R, g, trans='T')
+        # the inner solve only has to take eta down to tol: a tenth of that
+        # ratio, between inner_tol and 0.1
+        itol = min(0.1, max(inner_tol, 0.1 * tol / eta)) if eta > 0 else inner_tol
         dz = np.zeros(n)
         rv = rhs.copy()
         pv = rv.copy()
@@ -731,7 +735,7 @@ def spo_mixed(A, b, tol, rng, sampling_factor=4, npieces=4, inner_tol=1e-6, max_
             dz += alpha * pv
             rv -= alpha * q
             rr_new = rv @ rv
-            if rr_new <= inner_tol ** 2 * rr0:
+            if rr_new <= itol ** 2 * rr0:
                 break
             pv = rv + (rr_new / rr) * pv
             rr = rr_new

@@ -34,7 +34,7 @@ from parla_amd.comps.determiter.logging import SketchAndPrecondLog
 from parla_amd.drivers.least_squares import OverLstsqSolver, dim_checks
 
 
-def cg_normal(op, rhs, max_iter, tol, check_every=8):
+def cg_normal(op, rhs, max_iter, tol, check_every=4):
     """z ~= (B^T B)^{-1} rhs by conjugate gradients.  The recurrences (dots,
     alpha, beta, the z / r / p updates) run in one device kernel per step
     (pamd_bt_cg on a one-problem descriptor); the host reads ||r|| once every
@@ -141,7 +141,10 @@ class SPOMixed(OverLstsqSolver):
             if eta <= tol or (step > 0 and eta > 0.5 * etas[-2]) or step == self.max_refine:
                 break
             rhs = ops.trsv_(R, g, trans=True)
-            dz, its = cg_normal(normal, rhs, self.max_inner, self.inner_tol)
+            # the inner solve only has to take eta down to tol (or to the FP64
+            # floor): ask CG for a tenth of that ratio, between inner_tol and 0.1
+            inner = min(0.1, max(self.inner_tol, 0.1 * tol / eta)) if eta > 0 else self.inner_tol
+            dz, its = cg_normal(normal, rhs, self.max_inner, inner)
             inner_total += its
             x.add_(ops.trsv_(R, dz))
         x = best_x
