@@ -721,8 +721,8 @@ def lsqr_run(A, b, R=None, sqrt_delta=0.0, x0=None, damp=0.0, atol=1e-8, btol=1e
     """Device LSQR on [A; sqrt_delta I] R^{-1} with rhs [b; 0].
 
     Returns (x, var, arnorms, result) -- or (x, x_orig, var, arnorms, result)
-    with ``want_orig``, x_orig = R^{-1} x computed with the iteration's own
-    explicit inverse."""
+    with ``want_orig``, x_orig = R^{-1} x computed by the iteration's own
+    triangular substitution."""
     A = A if A.stride(-1) == 1 else A.contiguous()
     m, n = A.shape
     dev = A.device
