@@ -67,12 +67,20 @@ __device__ __forceinline__ void trsm_strip_body(int64_t strip, int64_t m, int k,
   for (int J = 0; J < nb; ++J) {
     const int j0 = J * TS_B;
     const int jc = j0 + lr;  // this lane's column of R / of the block
-    // partial of X_{<J} R_{<J,J}: wave w takes the k-steps kk = 4 (w + 4 q)
+    // partial of X_{<J} R_{<J,J}: wave w takes the k-steps kk = 4 (w + 4 q),
+    // in order; the R values of four steps are loaded before their MFMAs
     d4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int kk = 4 * wave; kk < j0; kk += 16) {
-      const double a = S[lr * lds + kk + lk];
-      const double b = (jc < k) ? R[(int64_t)(kk + lk) * ldr + jc] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    for (int kk = 4 * wave; kk < j0; kk += 64) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k2 = kk + 16 * u;
+        b[u] = (k2 < j0 && jc < k) ? R[(int64_t)(k2 + lk) * ldr + jc] : 0.0;
+        a[u] = (k2 < j0) ? S[lr * lds + k2 + lk] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (kk + 16 * u < j0) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) P[(wave * TS_B + lk + 4 * r) * 17 + lr] = acc[r];
