@@ -232,8 +232,15 @@ __device__ inline void gather_partials(const LsqrConst& K, double* ws, const Lsq
   for (int64_t c0 = 0; c0 < n; c0 += 64) {
     const int64_t c = c0 + lane;
     double t = 0.0;
-    if (c < n)
-      for (int g = wave; g < K.grid; g += nw) t += pv[(int64_t)g * n + c];
+    if (c < n)  // partials g = wave, wave + nw, ... in order, eight loads in flight
+      for (int g = wave; g < K.grid; g += 8 * nw) {
+        double pg[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pg[u] = (g + u * nw < K.grid) ? pv[(int64_t)(g + u * nw) * n + c] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (g + u * nw < K.grid) t += pg[u];
+      }
     cred[wave][lane] = t;
     __syncthreads();
     if (wave == 0 && c < n) {

@@ -33,6 +33,7 @@ __device__ inline void trsv_upper_block(int trans, int n, const double* __restri
     const int r0 = blk * 64;
     const int bs = min(64, n - r0);
     const int r1 = r0 + bs;
+#pragma unroll 4
     for (int e = tid; e < 64 * 64; e += blockDim.x) {
       const int i = e >> 6, j = e & 63;
       const double v = (i < bs && j < bs) ? R[(int64_t)(r0 + i) * ldr + r0 + j] : 0.0;
@@ -40,21 +41,42 @@ __device__ inline void trsv_upper_block(int trans, int n, const double* __restri
       if (i == j) rdiag[i] = (i < bs) ? 1.0 / v : 0.0;
     }
     if (!trans) {
-      // xs[r0:r1] -= R[r0:r1, r1:n] @ xs[r1:n]
+      // xs[r0:r1] -= R[r0:r1, r1:n] @ xs[r1:n]; a lane's terms in order, the
+      // row values of four of them loaded before they are used
       if (r1 < n) {
         for (int i = wave; i < bs; i += nwaves) {
           const double* row = R + (int64_t)(r0 + i) * ldr;
           double s = 0.0;
-          for (int j = r1 + lane; j < n; j += 64) s += row[j] * xs[j];
+          for (int j = r1 + lane; j < n; j += 256) {
+            double rv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rv[u] = (j + 64 * u < n) ? row[j + 64 * u] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (j + 64 * u < n) s += rv[u] * xs[j + 64 * u];
+          }
           s = wave_sum(s);
           if (lane == 0) xs[r0 + i] -= s;
         }
       }
     } else {
-      // xs[r0:r1] -= R[0:r0, r0:r1]^T @ xs[0:r0]
+      // xs[r0:r1] -= R[0:r0, r0:r1]^T @ xs[0:r0]; rows i = wave, + nwaves, ...
+      // in order, four rows' values loaded before they are used
       double s = 0.0;
       if (lane < bs)
-        for (int i = wave; i < r0; i += nwaves) s += R[(int64_t)i * ldr + r0 + lane] * xs[i];
+        for (int i = wave; i < r0; i += 4 * nwaves) {
+          double rv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int ii = i + u * nwaves;
+            rv[u] = (ii < r0) ? R[(int64_t)ii * ldr + r0 + lane] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int ii = i + u * nwaves;
+            if (ii < r0) s += rv[u] * xs[ii];
+          }
+        }
       red[wave][lane] = s;
     }
     __syncthreads();
