@@ -56,13 +56,13 @@ __device__ __forceinline__ uint16_t gauss_q_value(double z) {
 }
 
 // The quantised values of normal_pair(seed, pair): the pair is first formed
-// in FP32 from the same uniforms.  Its error is below 3e-4 in units of
-// 1/32: the uniforms rounded to FP32 (2^-24 relative), logf / sincospif
-// within 2 ulp, sqrtf and the products within 1 ulp give |z32 - z| <= 4e-6
-// + 6e-7 |z| where 32 z can reach a rounding boundary (|z| >= 1/64).  When
-// either 32 z32 lies within 1e-3 of a half-integer (about one pair in 250)
-// the pair is recomputed in FP64, so the result is always that of
-// gauss_q_value on the FP64 pair.
+// in FP32 from the same uniforms.  With the uniforms rounded to FP32 (2^-24
+// relative), logf and sincospif within 2 ulp and sqrtf and the products
+// within 1 ulp, T = -2 log u1 is off by at most 1.2e-7 + 2.4e-7 T, the radius
+// r = sqrt(T) by 6e-8 / r + 2.4e-7 r, cos / sin by 6.2e-7, and so
+// |z32 - z| <= 6e-8 / r + 9.8e-7 r.  When either 32 z32 lies within twice
+// that bound (in units of 1/32) of a half-integer, the pair is recomputed in
+// FP64, so the result is always that of gauss_q_value on the FP64 pair.
 __device__ __forceinline__ void normal_pair_q(uint64_t seed, uint64_t pair, uint16_t& q0, uint16_t& q1) {
   const U4 r = philox4x32_10(U4{(uint32_t)pair, (uint32_t)(pair >> 32), 0x6A09E667u, 0u}, (uint32_t)seed,
                              (uint32_t)(seed >> 32));
@@ -73,7 +73,8 @@ __device__ __forceinline__ void normal_pair_q(uint64_t seed, uint64_t pair, uint
   sincospif(2.0f * u2, &sf, &cf);
   const float t0 = 32.0f * (rad * cf), t1 = 32.0f * (rad * sf);
   const float d0 = fabsf(t0 - (floorf(t0) + 0.5f)), d1 = fabsf(t1 - (floorf(t1) + 0.5f));
-  if (d0 < 1e-3f || d1 < 1e-3f) {
+  const float margin = 64.0f * (6e-8f / fmaxf(rad, 1e-30f) + 1e-6f * rad);  // 2 x 32 x the bound
+  if (d0 < margin || d1 < margin) {
     double z0, z1;
     normal_pair(seed, pair, z0, z1);
     q0 = gauss_q_value(z0);
