@@ -36,9 +36,13 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int OZ_CHUNK = 256;       // K rows per exact FP32 accumulation
 constexpr int OZ_MAX_PIECES = 5;
 constexpr int OZ_TM = 128, OZ_TN = 64;
-constexpr int OZ_THREADS = 512;     // 8 waves, 4 (M) x 2 (N), 32 x 32 outputs each
+// 16-row blocks per wavefront: 4 (M) x 2 (N) wavefronts of 32 x 32 outputs.  Four
+// wavefronts of 64 x 32 read less LDS per MFMA but, one per SIMD, hide no
+// latency (1.6x slower at 4096 x 262144 x 1025).
+constexpr int OZ_WAVE_MB = 2;
+constexpr int OZ_THREADS = 64 * 2 * (OZ_TM / (16 * OZ_WAVE_MB));
 constexpr int OZ_TARGET_WG = 4096;  // K groups until the grid has about this many workgroups
-constexpr int OZ_BATCH = 2;         // K steps of 32 per LDS buffer (two buffers)
+constexpr int OZ_BAND = 8;          // row tiles per band of the tile order
 
 __device__ __forceinline__ uint16_t bf16_bits_of_int(double a) {
   // a is an integer with |a| <= 256: exact in BF16 (8 significant bits)
@@ -191,17 +195,26 @@ __global__ __launch_bounds__(256) void oz_split_kernel(int64_t m, int64_t n, con
 //                   sum_p 2^(e_cj - 8 - 9p) Q[i, chunk c] . a_p[chunk c, j]
 // and out = scale * sum_g part[g] (groups in order).  Q (rows padded to 128)
 // and the pieces (rows = columns of [A | b], padded to 64) are blocked
-// (blocked.h).  One 512-thread workgroup per 128 x 64 tile and K group; the
-// wavefronts of one XCD share their K group and column tile.
+// (blocked.h).  One 512-thread workgroup per 128 x 64 tile and K group.
 //
-// The K steps of 32 stream through LDS in batches of OZ_BATCH steps by direct
-// global -> LDS loads, double-buffered: batch j + 1 is loaded while batch j
-// is multiplied.  Direct-to-LDS loads of one wavefront can complete out of
-// issue order, so a partial vmcnt does not say which of them are in: every
-// batch starts with vmcnt(0) and a barrier, and no other global load is
-// waited on inside a batch (the chunk exponents for the next batch are loaded
-// with the next batch's blocks and consumed after the next vmcnt(0)).
+// The K steps of 32 stream through LDS in batches of oz_batch(P) steps by
+// direct global -> LDS loads, with the 64 chunk exponents of the tile's
+// columns beside each batch.  Direct-to-LDS loads of one wavefront can
+// complete out of issue order, so a partial vmcnt does not say which of them
+// are in; instead the two groups of four wavefronts own alternate batches.
+// The owner of batch j issues its loads two batches ahead and issues nothing
+// else until batch j, where it waits with vmcnt(0); a barrier then publishes
+// the batch to all eight wavefronts.  Three buffers keep two batches in
+// flight while one is multiplied.
 // ---------------------------------------------------------------------------
+constexpr int oz_batch(int P) { return P <= 4 ? 2 : 1; }  // three buffers of 8 + 4 P KiB per step fit in LDS
+
+template <int P>
+struct OzBuf {
+  uint4 blk[oz_batch(P)][OZ_TM / 16 + P * (OZ_TN / 16)][64];
+  int e[OZ_TN];
+};
+
 template <int P>
 __global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(int64_t kp, int64_t group_chunks, int mtiles,
                                                              int ntiles, int ngroups,
@@ -212,20 +225,30 @@ __global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(int64_t kp, int64_t
                                                              int64_t ncp, double* __restrict__ part) {
   constexpr int LB = OZ_TM / 16, RB = OZ_TN / 16;  // 1 KiB blocks per K step: Q rows, piece rows
   constexpr int NBLK = LB + P * RB;
-  constexpr int NW = OZ_THREADS / 64;
-  constexpr int B = OZ_BATCH;
+  constexpr int MB = OZ_WAVE_MB, WM = LB / MB;    // row blocks per wavefront, wavefronts along M
+  constexpr int G = 2, NWG = (OZ_THREADS / 64) / G;  // loading groups, wavefronts per group
+  static_assert(LB % NWG == 0 && RB % NWG == 0, "row blocks split evenly over a group");
+  constexpr int B = oz_batch(P);
   constexpr int CS = OZ_CHUNK / 32;  // K steps per chunk
   static_assert(CS % B == 0, "a batch lies within one chunk");
-  __shared__ uint4 SA[B][NBLK][64], SB[B][NBLK][64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 3, wn = wave >> 2;
+  __shared__ OzBuf<P> S[3];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: block choices are scalar
+  const int wm = wave % WM, wn = wave / WM;
+  const int grp = wave / NWG, wl = wave % NWG;
   const int lr = lane & 15, lk = lane >> 4;
   const int frag = blk_frag(lane);
   const uint32_t total = (uint32_t)mtiles * ntiles * ngroups;
   const uint32_t t = xcd_tile(blockIdx.x, total);
   const int g = (int)(t / ((uint32_t)mtiles * ntiles));
+  // within a K group, bands of OZ_BAND row tiles, column-major inside a band:
+  // the 32 workgroups an XCD runs at once share 8 Q row tiles and 4 column
+  // tiles of pieces in its L2 instead of 32 Q row tiles and one column tile
   const uint32_t rem = t % ((uint32_t)mtiles * ntiles);
-  const int nt = (int)(rem / mtiles), mt = (int)(rem % mtiles);
+  const uint32_t band = rem / ((uint32_t)OZ_BAND * ntiles);
+  const uint32_t bw = imin64(OZ_BAND, (int64_t)mtiles - (int64_t)band * OZ_BAND);
+  const uint32_t v = rem - band * (uint32_t)OZ_BAND * ntiles;
+  const int nt = (int)(v / bw), mt = (int)(band * OZ_BAND + v % bw);
   const int64_t kbn = kp >> 5;
   const int64_t nchunks = kp / OZ_CHUNK;
   const int64_t c0 = (int64_t)g * group_chunks;
@@ -235,77 +258,73 @@ __global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(int64_t kp, int64_t
   const uint4* Qg = reinterpret_cast<const uint4*>(Q);
   const uint4* Pg = reinterpret_cast<const uint4*>(pieces);
   const int64_t ps4 = pstride >> 3;
+  // this lane's exponent column for the exponent load (clamped: the padded
+  // columns have zero pieces, so any finite exponent leaves them zero)
+  const int64_t ecol = imin64((int64_t)nt * OZ_TN + lane, ncols - 1);
 
-  // wavefront w loads blocks w, w + NW, ... of every step of batch j
-  auto issue = [&](int j, uint4 (*dst)[NBLK][64]) {
+  // the calling wavefront (of the group owning batch j) loads rows wl,
+  // wl + NWG, ... of Q and of every piece for each step of the batch, and
+  // wavefront 0 of the group the batch's chunk exponents
+  auto issue = [&](int j, OzBuf<P>& dst) {
 #pragma unroll
     for (int u = 0; u < B; ++u) {
       const int64_t kb = kb0 + (int64_t)j * B + u;
 #pragma unroll
-      for (int i = 0; i < (NBLK + NW - 1) / NW; ++i) {
-        const int b = wave + NW * i;
-        if (b < NBLK) {
-          const uint4* src;
-          if (b < LB) {
-            src = Qg + (((int64_t)mt * LB + b) * kbn + kb) * 64;
-          } else {
-            const int p = (b - LB) / RB, rb = (b - LB) % RB;
-            src = Pg + p * ps4 + (((int64_t)nt * RB + rb) * kbn + kb) * 64;
-          }
-          blk_load_lds(src, &dst[u][b][0], lane);
-        }
+      for (int r = 0; r < LB / NWG; ++r) {
+        const int rb = wl + NWG * r;
+        blk_load_lds(Qg + (((int64_t)mt * LB + rb) * kbn + kb) * 64, &dst.blk[u][rb][0], lane);
       }
-    }
-  };
-  // exponents (this lane's two columns) of the chunk holding batch j
-  auto load_exps = [&](int j, int (&e)[2]) {
-    const int64_t cidx = c0 + ((int64_t)j * B) / CS;
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
-      const int64_t col = (int64_t)nt * OZ_TN + 32 * wn + 16 * nb + lr;
-      e[nb] = (col < ncols) ? exps[cidx * lde + col] : 0;
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int r = 0; r < RB / NWG; ++r) {
+          const int rb = wl + NWG * r;
+          blk_load_lds(Pg + p * ps4 + (((int64_t)nt * RB + rb) * kbn + kb) * 64, &dst.blk[u][LB + p * RB + rb][0],
+                       lane);
+        }
+    }
+    if (wl == 0) {
+      const int64_t cidx = c0 + ((int64_t)j * B) / CS;
+      __builtin_amdgcn_global_load_lds((blk_glb_void*)(exps + cidx * lde + ecol), (blk_lds_void*)&dst.e[0], 4, 0, 0);
     }
   };
 
-  f4 acc[P][2][2];
-  double acc64[2][2][4];
+  f4 acc[P][MB][2];
+  double acc64[MB][2][4];
 #pragma unroll
   for (int p = 0; p < P; ++p)
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < MB; ++a)
 #pragma unroll
       for (int bq = 0; bq < 2; ++bq) acc[p][a][bq] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < MB; ++a)
 #pragma unroll
     for (int bq = 0; bq < 2; ++bq)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc64[a][bq][r] = 0.0;
-  int ecur[2] = {0, 0}, enext[2] = {0, 0};
 
-  auto batch = [&](int j, uint4 (*cur)[NBLK][64], uint4 (*nxt)[NBLK][64]) {
-    __builtin_amdgcn_s_waitcnt(0x3F70);  // vmcnt(0): this batch's blocks and exponents are in
-    asm volatile("s_barrier" ::: "memory");  // ... for every wavefront; the other buffer is free
-    ecur[0] = enext[0];
-    ecur[1] = enext[1];
-    if (j + 1 < nbatch) {
-      issue(j + 1, nxt);
-      load_exps(j + 1, enext);
-    }
+  // batch j from buffer cur; ahead is the buffer read in batch j - 1, which
+  // the barrier frees for the owner's loads of batch j + 2
+  auto batch = [&](int j, const OzBuf<P>& cur, OzBuf<P>& ahead) {
+    const bool own = (j % G) == grp;
+    if (own) __builtin_amdgcn_s_waitcnt(0x3F70);  // vmcnt(0): this batch's blocks and exponents are in
+    asm volatile("s_barrier" ::: "memory");
+    if (own && j + G < nbatch) issue(j + G, ahead);
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-      bf16x8 a[2], bv[P][2];
+      bf16x8 a[MB], bv[P][2];
 #pragma unroll
-      for (int mb = 0; mb < 2; ++mb) a[mb] = *reinterpret_cast<const bf16x8*>(&cur[u][wm * 2 + mb][frag]);
+      for (int mb = 0; mb < MB; ++mb) a[mb] = *reinterpret_cast<const bf16x8*>(&cur.blk[u][wm * MB + mb][frag]);
 #pragma unroll
       for (int p = 0; p < P; ++p)
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb)
-          bv[p][nb] = *reinterpret_cast<const bf16x8*>(&cur[u][LB + p * RB + wn * 2 + nb][frag]);
+          bv[p][nb] = *reinterpret_cast<const bf16x8*>(&cur.blk[u][LB + p * RB + wn * 2 + nb][frag]);
 #pragma unroll
       for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int mb = 0; mb < 2; ++mb)
+        for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
           for (int nb = 0; nb < 2; ++nb)
             acc[p][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], bv[p][nb], acc[p][mb][nb], 0, 0, 0);
@@ -313,11 +332,12 @@ __global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(int64_t kp, int64_t
     if (((j + 1) * B) % CS == 0) {  // end of a chunk: exact FP32 sums -> FP64, pieces in order
 #pragma unroll
       for (int nb = 0; nb < 2; ++nb) {
+        const int e = cur.e[32 * wn + 16 * nb + lr];
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-          const double sc = ldexp(1.0, ecur[nb] - 8 - 9 * p);
+          const double sc = ldexp(1.0, e - 8 - 9 * p);
 #pragma unroll
-          for (int mb = 0; mb < 2; ++mb)
+          for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc64[mb][nb][r] += (double)acc[p][mb][nb][r] * sc;
         }
@@ -325,29 +345,28 @@ __global__ __launch_bounds__(OZ_THREADS) void oz_gemm_kernel(int64_t kp, int64_t
 #pragma unroll
       for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int mb = 0; mb < 2; ++mb)
+        for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
           for (int nb = 0; nb < 2; ++nb) acc[p][mb][nb] = f4{0.f, 0.f, 0.f, 0.f};
     }
   };
 
-  if (nbatch > 0) {
-    issue(0, SA);
-    load_exps(0, enext);
-  }
-  for (int j = 0; j < nbatch; j += 2) {
-    batch(j, SA, SB);
-    if (j + 1 < nbatch) batch(j + 1, SB, SA);
+  if (grp == 0 && nbatch > 0) issue(0, S[0]);
+  if (grp == 1 && nbatch > 1) issue(1, S[1]);
+  for (int j = 0; j < nbatch; j += 3) {
+    batch(j, S[0], S[2]);
+    if (j + 1 < nbatch) batch(j + 1, S[1], S[0]);
+    if (j + 2 < nbatch) batch(j + 2, S[2], S[1]);
   }
   // D layout of v_mfma_f32_16x16x32_bf16: lane l, item r -> row 4 (l / 16) + r, col l % 16
   double* w = part + (int64_t)g * d * ncp;
 #pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
+  for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t i = (int64_t)mt * OZ_TM + 32 * wm + 16 * mb + 4 * lk + r;
+        const int64_t i = (int64_t)mt * OZ_TM + 16 * (MB * wm + mb) + 4 * lk + r;
         const int64_t j = (int64_t)nt * OZ_TN + 32 * wn + 16 * nb + lr;
         if (i < d) w[i * ncp + j] = acc64[mb][nb][r];
       }
