@@ -275,16 +275,35 @@ __global__ __launch_bounds__(QR_REG_THREADS) void qr_panel_reg_kernel(int64_t m,
   __shared__ double taus[QR_NB];
   __shared__ double gred[NW][NG];
   __shared__ double Gp[QR_NPAIR];
+  // row slices of NT rows pass between global memory and the thread-per-row
+  // registers through LDS: globally, eight consecutive lanes move one row's
+  // QR_NB values, so a wavefront touches 8 rows per access instead of 64
+  __shared__ double stg[2][NT][QR_NB + 1];
+  static_assert(QR_NB == 8, "eight lanes per row in the staged copies");
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int64_t mr = m - c0;
+  const int sc = tid & 7, sr = tid >> 3;  // staged copies: column, first row (rows sr + 64 i)
   double a[RPT][QR_NB];
+  {
+    double ld[RPT][QR_NB];  // all global loads in flight at once
 #pragma unroll
-  for (int q = 0; q < RPT; ++q) {
-    const int64_t r = tid + NT * q;
+    for (int q = 0; q < RPT; ++q)
 #pragma unroll
-    for (int c = 0; c < QR_NB; ++c) a[q][c] = (r < mr && c < nb) ? A[(c0 + r) * lda + c0 + c] : 0.0;
+      for (int i = 0; i < QR_NB; ++i) {
+        const int64_t r = (int64_t)NT * q + sr + (NT / QR_NB) * i;
+        ld[q][i] = (r < mr && sc < nb) ? A[(c0 + r) * lda + c0 + sc] : 0.0;
+      }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+#pragma unroll
+      for (int i = 0; i < QR_NB; ++i) stg[0][sr + (NT / QR_NB) * i][sc] = ld[q][i];
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < QR_NB; ++c) a[q][c] = stg[0][tid][c];
+      __syncthreads();
+    }
   }
   const int idx8 = wave_sum_multi_index<QR_NB>(lane);
 #pragma unroll
@@ -361,23 +380,35 @@ __global__ __launch_bounds__(QR_REG_THREADS) void qr_panel_reg_kernel(int64_t m,
   double g[NG];
 #pragma unroll
   for (int p = 0; p < NG; ++p) g[p] = 0.0;
+  // indices of the staged store, opaque here: addresses derived from them are
+  // not precomputed before the column loop, where they would spill
+  const int wr = pamd_opaque(sr), wc = pamd_opaque(sc);
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     const int64_t r = tid + NT * q;
-    if (r < mr) {
-      double vr[QR_NB];
+    double vr[QR_NB];
 #pragma unroll
-      for (int c = 0; c < QR_NB; ++c) {
-        if (c < nb) A[(c0 + r) * lda + c0 + c] = a[q][c];
-        vr[c] = (c >= nb || c > r) ? 0.0 : ((c == r) ? 1.0 : a[q][c]);
-        V[r * QR_NB + c] = vr[c];
-      }
-      int p = 0;
-#pragma unroll
-      for (int x = 0; x < QR_NB; ++x)
-#pragma unroll
-        for (int y = x + 1; y < QR_NB; ++y) g[p++] += vr[x] * vr[y];
+    for (int c = 0; c < QR_NB; ++c) {
+      vr[c] = (c >= nb || c > r || r >= mr) ? 0.0 : ((c == r) ? 1.0 : a[q][c]);
+      stg[0][tid][c] = a[q][c];
+      stg[1][tid][c] = vr[c];
     }
+    int p = 0;
+#pragma unroll
+    for (int x = 0; x < QR_NB; ++x)
+#pragma unroll
+      for (int y = x + 1; y < QR_NB; ++y) g[p++] += vr[x] * vr[y];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < QR_NB; ++i) {
+      const int rl = wr + (NT / QR_NB) * i;
+      const int64_t rs = (int64_t)NT * q + rl;
+      if (rs < mr) {
+        if (wc < nb) A[(c0 + rs) * lda + c0 + wc] = stg[0][rl][wc];
+        V[rs * QR_NB + wc] = stg[1][rl][wc];
+      }
+    }
+    __syncthreads();
   }
   const double gsum = wave_sum_multi<NG>(g, lane);
   if (lane < NG) gred[wave][wave_sum_multi_index<NG>(lane)] = gsum;
