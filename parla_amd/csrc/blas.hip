@@ -213,7 +213,15 @@ __global__ void splitk_reduce_kernel(int64_t M, int64_t N, int splits, double al
 
 int gemm_splits(int64_t M, int64_t N, int64_t K) {
   const int64_t tiles = pamd_cdiv(M, gemm_edge(M)) * pamd_cdiv(N, gemm_edge(N));
-  if (K < 4 * 256) return 1;
+  if (K < 4 * 256) {
+    // a short K: split only an output of few tiles (the k x k products of
+    // CholeskyQR at k = 266: 9 tiles, 36 -> 18 us with 64 K per slice); a
+    // larger output fills the chip without, and its partials would cost more
+    // than they save (8192 x 266 x 266: 43 us unsplit, 49 us in 3 slices)
+    if (tiles >= 128) return 1;
+    const int64_t s = imin64(pamd_cdiv(256, tiles), K / 64);
+    return (int)imax64(1, imin64(s, 64));
+  }
   // a large output with a long K (the power iteration's A S, A^T Y and Q^T A
   // at 8192 x 4096, k = 266): split K until there are about 3072 workgroups,
   // twelve per CU, so enough wavefronts are resident to hide MFMA and load

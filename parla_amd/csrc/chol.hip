@@ -537,18 +537,34 @@ __global__ __launch_bounds__(SC_THREADS) void small_combine_kernel(int k, const 
                                                                   double alpha, double sigma, int upper_half,
                                                                   double* __restrict__ out, int64_t ldo,
                                                                   double* __restrict__ nrm) {
+  constexpr int U = 8;  // elements per thread whose loads are in flight together
   __shared__ double red[SC_THREADS / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double ss = 0.0;
-  const int64_t total = (int64_t)k * k;
-  for (int64_t e = tid; e < total; e += SC_THREADS) {
-    const int i = (int)(e / k), j = (int)(e % k);
-    double v = A[i * lda + j];
-    if (B != nullptr) v += alpha * B[i * ldb + j];
-    if (i == j) v += sigma;
-    ss += v * v;
-    if (upper_half) v = (j > i) ? v : ((j == i) ? 0.5 * v : 0.0);
-    out[i * ldo + j] = v;
+  const int total = k * k;  // k <= 512
+  // thread tid takes the elements tid, tid + SC_THREADS, ... in that order
+  for (int base = 0; base < total; base += SC_THREADS * U) {
+    double va[U], vb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = base + tid + SC_THREADS * u;
+      const int i = e / k, j = e - (e / k) * k;
+      va[u] = (e < total) ? A[(int64_t)i * lda + j] : 0.0;
+      vb[u] = (e < total && B != nullptr) ? B[(int64_t)i * ldb + j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = base + tid + SC_THREADS * u;
+      if (e < total) {
+        const int i = e / k, j = e - (e / k) * k;
+        double v = va[u];
+        if (B != nullptr) v += alpha * vb[u];
+        if (i == j) v += sigma;
+        ss += v * v;
+        if (upper_half) v = (j > i) ? v : ((j == i) ? 0.5 * v : 0.0);
+        out[(int64_t)i * ldo + j] = v;
+      }
+    }
   }
   if (nrm != nullptr) {
     ss = wave_sum(ss);
@@ -558,6 +574,37 @@ __global__ __launch_bounds__(SC_THREADS) void small_combine_kernel(int k, const 
       double t = 0.0;
       for (int w = 0; w < SC_THREADS / 64; ++w) t += red[w];
       nrm[0] = sqrt(t);
+    }
+  }
+}
+
+// the same without the norm, over many workgroups: 8 elements per thread
+constexpr int SCW_PER_BLOCK = 256 * 8;
+__global__ __launch_bounds__(256) void small_combine_wide_kernel(int k, const double* __restrict__ A, int64_t lda,
+                                                                 const double* __restrict__ B, int64_t ldb,
+                                                                 double alpha, double sigma, int upper_half,
+                                                                 double* __restrict__ out, int64_t ldo) {
+  constexpr int U = SCW_PER_BLOCK / 256;
+  const int total = k * k;  // k <= 512
+  const int base = (int)blockIdx.x * SCW_PER_BLOCK + (int)threadIdx.x;
+  double va[U], vb[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int e = base + 256 * u;
+    const int i = e / k, j = e - (e / k) * k;
+    va[u] = (e < total) ? A[(int64_t)i * lda + j] : 0.0;
+    vb[u] = (e < total && B != nullptr) ? B[(int64_t)i * ldb + j] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int e = base + 256 * u;
+    if (e < total) {
+      const int i = e / k, j = e - (e / k) * k;
+      double v = va[u];
+      if (B != nullptr) v += alpha * vb[u];
+      if (i == j) v += sigma;
+      if (upper_half) v = (j > i) ? v : ((j == i) ? 0.5 * v : 0.0);
+      out[(int64_t)i * ldo + j] = v;
     }
   }
 }
@@ -715,8 +762,13 @@ extern "C" int pamd_small_combine_f64(int64_t k, const double* A, int64_t lda, c
   PAMD_ARG_CHECK(out != nullptr || k == 0, 9);
   PAMD_ARG_CHECK(ldo >= k && ldo >= 1, 10);
   if (k == 0 && nrm == nullptr) return PAMD_OK;
-  small_combine_kernel<<<1, SC_THREADS, 0, (hipStream_t)stream>>>((int)k, A, lda, B, ldb, alpha, sigma, upper_half,
-                                                                 out, ldo, nrm);
+  if (nrm == nullptr) {  // no reduction: spread over the chip
+    small_combine_wide_kernel<<<(unsigned)pamd_cdiv(k * k, SCW_PER_BLOCK), 256, 0, (hipStream_t)stream>>>(
+        (int)k, A, lda, B, ldb, alpha, sigma, upper_half, out, ldo);
+  } else {
+    small_combine_kernel<<<1, SC_THREADS, 0, (hipStream_t)stream>>>((int)k, A, lda, B, ldb, alpha, sigma, upper_half,
+                                                                   out, ldo, nrm);
+  }
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }
