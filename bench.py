@@ -83,8 +83,8 @@ def main():
     args = ap.parse_args()
     legs = set(args.legs.split(","))
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X and has no "
-              "multi-GPU path (see DESIGN.md for how the workload would shard).", file=sys.stderr)
+        print("bench.py measures one GPU: parla_amd runs on a single MI355X and has no "
+              "multi-GPU path; run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch

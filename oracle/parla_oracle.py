@@ -766,7 +766,7 @@ def bf16_pieces(X, npieces):
     """The BF16 pieces of FP64 values, as uint16 bits, shape (npieces,) + X.shape:
     piece p is the BF16 rounding (through FP32, ties to even) of X minus the
     earlier pieces, the difference taken exactly in FP64.  No reference
-    counterpart (the split-precision QB products of this build).  Pinned by
+    counterpart (the split-precision QB products of parla_amd).  Pinned by
     the reconstruction bound |X - sum| <= 2^-(8 npieces) |X| and by
     bf16_round (tests/test_oracle.py)."""
     r = np.array(X, dtype=np.float64, copy=True)
