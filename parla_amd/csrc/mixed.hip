@@ -653,7 +653,7 @@ __global__ __launch_bounds__(64 * RED_WAVES) void residual_reduce_kernel(int64_t
 }
 
 // ---------------------------------------------------------------------------
-// TRSM leaf: X (m x nb, nb <= 64) <- X R^{-1}, R upper nb x nb; thread per row
+// TRSM leaf: X (m x nb, nb <= LEAF) <- X R^{-1}, R upper nb x nb; thread per row
 // ---------------------------------------------------------------------------
 constexpr int LEAF = 32;
 
