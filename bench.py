@@ -80,6 +80,10 @@ def main():
     ap.add_argument("--pmc-gemm", action="store_true",
                     help="only run the three QB GEMM shapes once each plus a 512 MiB calibration copy "
                          "(the workload of the rocprofv3 FETCH_SIZE / WRITE_SIZE passes)")
+    ap.add_argument("--pmc-bf16", action="store_true",
+                    help="only run the two BF16 MFMA GEMMs once each: the split GEMM on the QB1 shape "
+                         "A S (8192 x 266, K 4096, 2 pieces) and the exact sketch at A 262144 x 1024 "
+                         "(d 4096, 4 pieces) (the workload of the rocprofv3 SQ counter passes)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
     if args.gpus != 1:
@@ -104,6 +108,19 @@ def main():
         ops.gemm(Y.t(), A)          # B = Q^T A
         x = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
         x.clone()                   # calibration: 512 MiB read, 512 MiB written
+        torch.cuda.synchronize()
+        return
+    if args.pmc_bf16:
+        torch.manual_seed(0)
+        A = torch.randn(8192, 4096, dtype=torch.float64, device=dev)
+        S = torch.randn(4096, 266, dtype=torch.float64, device=dev)
+        kp, mpad, npad, _ = ops.split_layout(2, 8192, 266, 4096)
+        ops.split_gemm(ops.split_bf16(A, 2), ops.split_bf16(S, 2, transpose=True, rows_out=npad))
+        del A, S
+        A = torch.randn(262144, 1024, dtype=torch.float64, device=dev)
+        b = torch.randn(262144, dtype=torch.float64, device=dev)
+        Q = ops.gauss_q(4096, 262144, 1, dev)
+        ops.ozaki_sketch(Q, 4096, ops.OzakiSplit(A, b, 4), 1.0)
         torch.cuda.synchronize()
         return
 
