@@ -19,10 +19,10 @@
 // serves A S (L = A) and A^T Y (L = A^T) and each tall k-column factor is
 // split straight into its transpose.
 //
-// GEMM: one 256-thread workgroup per 128 x TN output tile and K range; the
-// four wavefronts split the tile 2 x 2 (P = 2, TN = 288) or 4 x 1 (P = 3,
-// TN = 144).  K advances 32 at a time through double-buffered LDS, with the
-// global loads two steps ahead in two register banks.  Skinny products
+// GEMM: one workgroup per 128 x TN output tile and K range; eight wavefronts
+// split the tile 4 x 2 (P = 2, TN = 288) or four split it 4 x 1 (P = 3,
+// TN = 144).  K advances 32 at a time through three LDS buffers filled by
+// direct global -> LDS loads two steps ahead (sg_gemm_kernel).  Skinny products
 // (N ~ 300) have few tiles, so K is split until there are at least 256
 // workgroups; the splits' FP32 partials are summed in split order in FP64 by
 // a second kernel, which also writes C in either orientation.  Tiles are
@@ -38,7 +38,6 @@ namespace {
 
 constexpr int SG_TM = 128;
 constexpr int SG_BK = 32;
-constexpr int SG_THREADS = 256;
 constexpr int SG_KALIGN = 64;
 constexpr int SG_NALIGN = 288;
 
@@ -130,23 +129,23 @@ __global__ __launch_bounds__(256) void sg_split_t_kernel(int64_t rows, int64_t c
 // instruction, no registers involved), two steps ahead.  Direct-to-LDS loads
 // of one wavefront can complete out of issue order, so only vmcnt(0) says
 // that a wavefront's loads are in.  The steps therefore belong alternately to
-// two groups of two wavefronts: the group that owns step s loads all of its
-// blocks, and at step s it waits with vmcnt(0) for them alone (its next
-// loads, for step s + 2, are issued after that wait), then the barrier makes
-// them visible to all.  Two steps stay in flight, and every wait is a full
-// one.  The buffers are separate arrays and the loop is unrolled by three,
+// two groups of wavefronts (the halves of the workgroup): the group that owns
+// step s loads all of its blocks, and at step s it waits with vmcnt(0) for
+// them alone (its next loads, for step s + 2, are issued after that wait),
+// then the barrier makes them visible to all.  Two steps stay in flight, and
+// every wait is a full one.  The buffers are separate arrays and the loop is unrolled by three,
 // so every access names its buffer at compile time.
 template <int P, int WM, int WN, int MB, int NB>
-__global__ __launch_bounds__(SG_THREADS) void sg_gemm_kernel(int64_t kp, int64_t kc, int mtiles, int ntiles,
+__global__ __launch_bounds__(64 * WM * WN) void sg_gemm_kernel(int64_t kp, int64_t kc, int mtiles, int ntiles,
                                                              int nsplit, const uint16_t* __restrict__ L,
                                                              int64_t lps, const uint16_t* __restrict__ R,
                                                              int64_t rps, int64_t M, float* __restrict__ ws,
                                                              int64_t npad) {
   constexpr int TM = WM * MB * 16, TN = WN * NB * 16;
-  static_assert(TM == SG_TM && WM * WN == 4, "four wavefronts on a 128-row tile");
+  static_assert(TM == SG_TM && (WM * WN == 4 || WM * WN == 8), "four or eight wavefronts on a 128-row tile");
   constexpr int LB = TM / 16, RB = TN / 16;  // 1 KiB blocks per piece per K step
   constexpr int NBLK = P * (LB + RB);        // blocks per K step: L pieces, then R pieces
-  constexpr int G = 2, NWG = 4 / G;          // loading groups, wavefronts per group
+  constexpr int G = 2, NWG = WM * WN / G;    // loading groups, wavefronts per group
   static_assert(LB % NWG == 0, "row blocks of L split evenly over a group");
   __shared__ uint4 S0[NBLK][64], S1[NBLK][64], S2[NBLK][64];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -391,11 +390,14 @@ extern "C" int pamd_split_gemm_f64(int npieces, int64_t M, int64_t N, int64_t K,
   const int64_t total = (int64_t)s.mtiles * s.ntiles * s.nsplit;
   PAMD_ARG_CHECK(total < (1LL << 31), 2);
   float* ws = (float*)workspace;
+  // 2 pieces: eight wavefronts of 32 x 144 (two per SIMD) beat four of
+  // 64 x 144 by 5-10% (QB1 A S 75.8 -> 71.7 us); 3 pieces: four of 32 x 144
+  // and eight of 16 x 144 are within 1%, four kept
   if (npieces == 2)
-    sg_gemm_kernel<2, 2, 2, 4, 9><<<(unsigned)total, SG_THREADS, 0, st>>>(
+    sg_gemm_kernel<2, 4, 2, 2, 9><<<(unsigned)total, 512, 0, st>>>(
         s.kp, s.kc, s.mtiles, s.ntiles, s.nsplit, L, lps, R, rps, M, ws, s.npad);
   else
-    sg_gemm_kernel<3, 4, 1, 2, 9><<<(unsigned)total, SG_THREADS, 0, st>>>(
+    sg_gemm_kernel<3, 4, 1, 2, 9><<<(unsigned)total, 256, 0, st>>>(
         s.kp, s.kc, s.mtiles, s.ntiles, s.nsplit, L, lps, R, rps, M, ws, s.npad);
   PAMD_LAUNCH_CHECK();
   dim3 rgrid((unsigned)pamd_cdiv(M, 64), (unsigned)pamd_cdiv(N, 64));
