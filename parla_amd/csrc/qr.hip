@@ -254,9 +254,11 @@ __device__ __forceinline__ int wave_sum_multi_index(int lane) {
 
 // Same factorisation with the panel held in registers: 512 threads, thread t
 // owns the panel rows t + 512 q (q < RPT).  Each column costs one fused
-// reduction (sum of squares + dot products, 8 values) and two barriers; the
-// shared values of a column are double-buffered by its parity, so no barrier
-// is needed between columns.  No memory traffic until the end.
+// reduction (sum of squares + dot products, 8 values) and one barrier, after
+// which every wavefront adds the waves' partials itself; the shared values of
+// a column are double-buffered by its parity, so no barrier is needed between
+// columns.  Global memory is touched only by the staged loads at the start and
+// the staged stores at the end.
 constexpr int QR_REG_THREADS = 512;
 
 template <int RPT>
@@ -271,7 +273,6 @@ __global__ __launch_bounds__(QR_REG_THREADS) void qr_panel_reg_kernel(int64_t m,
   static_assert(QR_NPAIR <= NG, "pairs fit the padded reduction");
   __shared__ double red[2][NW][QR_NB];
   __shared__ double prow[2][QR_NB];
-  __shared__ double tots[2][QR_NB];
   __shared__ double taus[QR_NB];
   __shared__ double gred[NW][NG];
   __shared__ double Gp[QR_NPAIR];
@@ -330,17 +331,15 @@ __global__ __launch_bounds__(QR_REG_THREADS) void qr_panel_reg_kernel(int64_t m,
     }
     const double wsum = wave_sum_multi<QR_NB>(part, lane);
     if (lane < QR_NB) red[buf][wave][idx8] = wsum;
-    __syncthreads();
-    if (tid < QR_NB) {  // one lane per value sums the waves in order
-      double sum = 0.0;
+    __syncthreads();  // the one barrier of the column: red and prow are double-buffered by parity
+    // every wavefront forms the totals itself: lane l sums value l % QR_NB over
+    // the waves in wave order, then the eight totals are broadcast in the wave
+    double mine = 0.0;
 #pragma unroll
-      for (int w = 0; w < NW; ++w) sum += red[buf][w][tid];
-      tots[buf][tid] = sum;
-    }
-    __syncthreads();
+    for (int w = 0; w < NW; ++w) mine += red[buf][w][lane & (QR_NB - 1)];
     double tot[QR_NB];
 #pragma unroll
-    for (int c = 0; c < QR_NB; ++c) tot[c] = tots[buf][c];
+    for (int c = 0; c < QR_NB; ++c) tot[c] = wave_bcast(mine, c);
     const double alpha = prow[buf][j];
     double beta, tau, scale;
     if (tot[0] == 0.0) {
