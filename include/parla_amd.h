@@ -84,8 +84,9 @@ int pamd_nrm2_workspace(int64_t n, int64_t* bytes);
 int pamd_nrm2_large_f64(int64_t n, const double* x, double* result,
                         void* workspace, void* stream);
 
-/* In-place triangular solve with the upper-triangular n x n R:
- * trans = 0: x <- R^{-1} x;  trans = 1: x <- R^{-T} x.
+/* In-place triangular solve with the upper-triangular n x n R (any n < 2^30;
+ * up to 2048 the solution vector is staged in LDS, beyond that it is solved
+ * in place in x): trans = 0: x <- R^{-1} x;  trans = 1: x <- R^{-T} x.
  * Replaces la.solve_triangular at drivers/least_squares.py:316,363 and
  * comps/preconditioning.py:28,37,40-41. */
 int pamd_trsv_upper_f64(int trans, int64_t n, const double* R, int64_t ldr,
@@ -103,6 +104,18 @@ int pamd_trsv_upper_f64(int trans, int64_t n, const double* R, int64_t ldr,
 int pamd_gaussian_fill_f64(int64_t rows, int64_t cols, double scale,
                            uint64_t seed, uint64_t offset, double* X,
                            int64_t ld, void* stream);
+
+/* The subsampled randomized cosine transform as a dense d x N operator:
+ * out[i][j] = s(r_i) cos(pi r_i (2 q + 1) / (2N)) e[q], q = pinv[j],
+ * s(0) = 1/sqrt(N), s(k) = sqrt(2/N) -- row r_i of the orthonormal DCT-II
+ * (scipy.fft.dct(norm='ortho')) composed with the sign vector e and the
+ * permutation perm (pinv = argsort(perm)), so out @ X equals apply_srct of
+ * utils/sketching.py:119-176 (mat[perm] * e, dct along axis 0, rows r), and
+ * out^T its adjoint.  The DCT argument is reduced exactly in integers
+ * before the cosine.  N < 2^30.  Draws come from generate_srct
+ * (utils/sketching.py:106-116), made on the host in the reference's order. */
+int pamd_srct_fill_f64(int64_t d, int64_t N, const int64_t* r, const double* e,
+                       const int64_t* pinv, double* out, int64_t ld, void* stream);
 
 /* Draw a wide SJLT W (n_short x n_long) on the device: column j gets k
  * distinct rows (k <= n_short; with replacement when n_short < k) and signs
@@ -179,6 +192,14 @@ int pamd_lu_pl_f64(int64_t m, int64_t n, const double* A, int64_t lda,
  * zero on its diagonal (lives inside `workspace`; host call, no GPU work). */
 int pamd_qr_singular_flag(const void* workspace, int64_t m, int64_t n,
                           int64_t ncols, int** flag_ptr);
+
+/* out = A diag(s) (invert = 0) or A diag(s)^{-1} (invert = 1), A rows x cols
+ * (out may alias A).  Forms the SVD right preconditioner M = V / sigma of
+ * comps/preconditioning.py:77 and its column-scaled variants
+ * (comps/determiter/saddle.py:138-143). */
+int pamd_scale_cols_f64(int64_t rows, int64_t cols, const double* A, int64_t lda,
+                        const double* s, int invert, double* out, int64_t ldo,
+                        void* stream);
 
 /* out (cols x rows) = A^T for A (rows x cols); plumbing for A.T @ S products. */
 int pamd_transpose_f64(int64_t rows, int64_t cols, const double* A, int64_t lda,
@@ -269,7 +290,8 @@ int pamd_lsqr_workspace(int64_t m, int64_t n, int64_t iter_lim, int64_t* bytes);
  * if calc_var), arnorms (iter_lim; the per-iteration ||A_pc^T r|| log of
  * lsqr.py:413) and fills *result (host).  All recurrences run on the device;
  * this call blocks, reading the device state back once per batch of
- * iterations. */
+ * iterations.  Any n < 2^30: up to n = 1900 the triangular solves keep their
+ * vector in LDS, beyond that in the workspace (slower, same arithmetic). */
 int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, double* x_orig,
                       double* var, double* arnorms, pamd_lsqr_result* result,
                       void* workspace, void* stream);
@@ -387,6 +409,15 @@ int pamd_mlp_adam(float* w, float* m, float* v, const float* grad, float lr,
 int pamd_row_gather_f64(int64_t d, int64_t m, int64_t n, const int64_t* idx,
                         const double* scale, const double* A, int64_t lda,
                         double* out, int64_t ldo, void* stream);
+
+/* out[idx[r], :] = A[r, :] for r < d (out has m rows, the other rows are left
+ * as they are; the indices must be distinct, 0 <= idx[r] < m): the adjoint of
+ * the row selection, S^T @ Y for the wide SkOpIN operator and S @ X for the
+ * tall one (comps/sketchers/oblivious.py:75-82, utils/sketching.py:204-232,
+ * whose rmatvec this is). */
+int pamd_row_scatter_f64(int64_t d, int64_t m, int64_t n, const int64_t* idx,
+                         const double* A, int64_t lda, double* out, int64_t ldo,
+                         void* stream);
 
 /* Batched potrf / trtri: problem i is the n[i] x n[i] matrix (ld n[i]) at
  * element offset off[i] of the pool (n and off are device arrays; n[i] <= 512

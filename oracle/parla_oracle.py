@@ -53,6 +53,9 @@ __all__ = [
     "residual_extended", "residual_exact", "backward_error_extended", "spo_mixed", "bf16_round",
     "mlp_unpack", "mlp_effective", "mlp_forward", "mlp_loss_grad", "leverage_scores", "coherent_matrix",
     "MLP_NPARAM", "bf16_pieces", "bf16_values", "split_product",
+    "svd_right_precond", "pcss2_under", "spu1", "sso1", "pcg", "pcss1", "pcss2_saddle", "sps1", "sps2",
+    "project_out", "qb2", "qb3", "evd2", "sparse_sign_operator", "generate_srct", "apply_srct",
+    "srct_matrix",
 ]
 
 
@@ -113,6 +116,68 @@ def sjlt_operator(n_rows, n_cols, rng, vec_nnz=8):
                           shape=(n_rows, n_cols))
         return S.tocsc()
     return sjlt_operator(n_cols, n_rows, rng, vec_nnz).T.tocsr()
+
+
+def sparse_sign_operator(n_rows, n_cols, rng, density=0.05):
+    """utils/sketching.py:83-103 as a dense array (same numpy draws)."""
+    rng = np.random.default_rng(rng)
+    nz = rng.random(n_rows * n_cols) < density
+    attempt = 0
+    while np.all(~nz):
+        if attempt == 10:
+            raise RuntimeError('Density too low.')
+        nz = rng.random(n_rows * n_cols) < density
+        attempt += 1
+    rows, cols = np.unravel_index(np.where(nz)[0], (n_rows, n_cols))
+    vals = np.ones(rows.size)
+    vals[rng.random(vals.size) < 0.5] = -1
+    vals /= np.sqrt(min(n_rows, n_cols) * density)
+    S = np.zeros((n_rows, n_cols))
+    S[rows, cols] = vals
+    return S
+
+
+def generate_srct(n_rows, n_cols, rng):
+    """utils/sketching.py:106-116: (r, e, perm) in the reference's draw order."""
+    big, small = max(n_rows, n_cols), min(n_rows, n_cols)
+    r = rng.choice(big, size=small, replace=False)
+    e = rng.random(big)
+    e[e > 0.5] = 1.0
+    e[e != 1] = -1.0
+    e *= np.sqrt(big / small)
+    perm = rng.permutation(big)
+    return r, e, perm
+
+
+def apply_srct(r, e, mat, perm=None, forward=True):
+    """utils/sketching.py:119-176 with scipy.fft's orthonormal DCT-II / its inverse."""
+    from scipy.fft import dct, idct
+    mat = np.asarray(mat, dtype=np.float64)
+    if forward:
+        if perm is not None:
+            mat = mat[perm]
+        mat = mat * (e[:, None] if mat.ndim > 1 else e)
+        return dct(mat, axis=0, norm="ortho")[r]
+    shape = (e.size,) if mat.ndim == 1 else (e.size, mat.shape[1])
+    out = np.zeros(shape)
+    out[r] = mat
+    out = idct(out, axis=0, norm="ortho")
+    out = out * (e[:, None] if out.ndim > 1 else e)
+    if perm is not None:
+        out = out[np.argsort(perm)]
+    return out
+
+
+def srct_matrix(n_rows, n_cols, rng):
+    """utils/sketching.py:179-201 as a dense array: the SRCT applied to the
+    identity (wide), or the transpose of a wide operator (tall)."""
+    rng = np.random.default_rng(rng)
+    r, e, perm = generate_srct(n_rows, n_cols, rng)
+    if n_cols < n_rows:
+        # the reference draws (r, e, perm) before it looks at the orientation and
+        # the tall operator transposes a second, wide draw (sketching.py:185-200)
+        return srct_matrix(n_cols, n_rows, rng).T
+    return apply_srct(r, e, np.eye(n_cols), perm)
 
 
 def sampling_operator(n_rows, n_cols, rng, indices=None):
@@ -406,12 +471,14 @@ def _sketch_gen(kind, vec_nnz=8, normalize=True):
         return lambda r, c, rng: gaussian_operator(r, c, rng, normalize)
     if kind == "sjlt":
         return lambda r, c, rng: sjlt_operator(r, c, rng, vec_nnz)
+    if kind == "srct":
+        return srct_matrix
     raise ValueError(kind)
 
 
 def spo(A, b, delta, tol, iter_lim, rng, sketch="gaussian", sampling_factor=4,
         mode="qr", logging=True, vec_nnz=8, return_info=False):
-    """SPO.__call__, drivers/least_squares.py:291-369 (modes 'qr' and 'chol').
+    """SPO.__call__, drivers/least_squares.py:291-369 (modes 'qr', 'chol', 'svd').
 
     Returns ``(x, log)``; with ``return_info`` also a dict holding the
     presolve decision, R, z_ske and the raw LSQR tuple.
@@ -448,6 +515,17 @@ def spo(A, b, delta, tol, iter_lim, rng, sketch="gaussian", sampling_factor=4,
         b_ske = S @ b
         z_ske = la.solve_triangular(R, A_ske.T @ b_ske, lower=False, trans="T")
         x_ske = la.solve_triangular(R, z_ske, lower=False)
+    elif mode == "svd":
+        # least_squares.py:330-339: R is the n x rank right preconditioner V / sigma
+        tic = clock()
+        if delta > 0:
+            A_ske = np.vstack((A_ske, sqrt_delta * np.eye(n)))
+        R, U, sigma, Vh = svd_right_precond(A_ske)
+        log.time_factor = clock() - tic
+        tic = clock()
+        b_ske = S @ b
+        z_ske = U[:d, :].T @ b_ske
+        x_ske = R @ z_ske
     else:
         raise ValueError()
 
@@ -462,11 +540,15 @@ def spo(A, b, delta, tol, iter_lim, rng, sketch="gaussian", sampling_factor=4,
     log.time_presolve = clock() - tic
 
     tic = clock()
-    x, y, arnorms, raw = pcss2(A, b, delta, tol, iter_lim, R, True, z_ske)
+    tri = mode in ("qr", "chol")
+    x, y, arnorms, raw = pcss2(A, b, delta, tol, iter_lim, R, tri, z_ske)
     log.time_iterate = clock() - tic
 
     if logging:
-        ar0 = la.solve_triangular(R, A.T @ b, "T", lower=False)
+        if tri:
+            ar0 = la.solve_triangular(R, A.T @ b, "T", lower=False)
+        else:
+            ar0 = R.T @ (A.T @ b)
         log.wrap_up(arnorms, la.norm(ar0))
     if return_info:
         info = dict(R=R, z_ske=z_ske, x_ske=x_ske, rel_err=rel_err,
@@ -874,3 +956,334 @@ def loglinear_fit(x, y):
 def loglog_fit(x, y):
     """utils/stats.py:31-56: fit log(y) = a + b log(x)."""
     return loglinear_fit(np.log(np.asarray(x, dtype=float)), y)
+
+
+# --------------------------------------------------------------------------
+# The drivers next to the hot path: SVD preconditioner, the underdetermined
+# branch, sketch-and-solve, PCG, saddle-point systems, blocked QB, Nystrom
+# EVD.  Pinned by the reference-written fixtures (SPO 'chol'/'svd', PcSS2
+# with c != 0 and delta > 0, SSO1, PcSS1, SPS1/SPS2, QB3, EVD2) and by closed
+# forms where the reference no longer runs under numpy 2 (QB2, SPU1).
+# --------------------------------------------------------------------------
+
+def svd_right_precond(A_ske):
+    """comps/preconditioning.py:70-79: M = V / sigma over the numerical rank
+    (sigma > sigma_1 * ncols * eps); returns (M, U, sigma, Vh)."""
+    U, sigma, Vh = la.svd(A_ske, full_matrices=False)
+    eps = np.finfo(float).eps
+    rank = np.count_nonzero(sigma > sigma[0] * A_ske.shape[1] * eps)
+    Vh = Vh[:rank, :]
+    U = U[:, :rank]
+    sigma = sigma[:rank]
+    return Vh.T / sigma, U, sigma, Vh
+
+
+class _Transposed:
+    """The adjoint of an operator with matvec / rmatvec (A_pc.T)."""
+
+    def __init__(self, op):
+        self.op = op
+        self.shape = (op.shape[1], op.shape[0])
+
+    def matvec(self, x):
+        return self.op.rmatvec(x)
+
+    def rmatvec(self, y):
+        return self.op.matvec(y)
+
+
+def pcss2_under(A, c, delta, tol, iter_lim, R, upper_tri):
+    """Underdetermined branch of PcSS2.__call__, saddle.py:203-214: LSQR on
+    A_pc^T with right-hand side M^T c.  With delta = 0 the reference writes
+    ``np.NaN * np.empty(n)`` for x (an attribute numpy 2 removed); NaN is
+    the evident value."""
+    m, n = A.shape
+    A_pc, _, M_adj = a_lift_precond(A, delta, R, upper_tri)
+    c_pc = M_adj(c)
+    res = lsqr(_Transposed(A_pc), c_pc, atol=tol, btol=tol, iter_lim=iter_lim)
+    y = res[0]
+    if delta > 0:
+        y = y[:m]
+        x = (A.T @ y - c) / delta
+    else:
+        x = np.full(n, np.nan)
+    return x, y, res[7], res
+
+
+def spu1(A, c, tol, iter_lim, rng, sketch="sjlt", sampling_factor=3, vec_nnz=8):
+    """SPU1.__call__, drivers/least_squares.py:464-494: min ||y|| s.t. A^T y = c
+    from the SVD preconditioner of S A and LSQR on (A M)^T."""
+    m, n = A.shape
+    d = dim_checks(sampling_factor, m, n)
+    rng = np.random.default_rng(rng)
+    log = SketchAndPrecondLog()
+    S = _sketch_gen(sketch, vec_nnz)(d, m, rng)
+    A_ske = S @ A
+    M = svd_right_precond(A_ske)[0]
+    x, y, errors, _ = pcss2_under(A, c, 0.0, tol, iter_lim, M, False)
+    log.wrap_up(errors, la.norm(A @ (M @ (M.T @ c))))
+    return y, log
+
+
+def sso1(A, b, delta, rng, sketch="sjlt", sampling_factor=3, vec_nnz=8, lapack_driver="gelsd"):
+    """SSO1.__call__, drivers/least_squares.py:162-189: solve the sketched
+    problem [S A; sqrt(delta) I] x = [S b; 0] by LAPACK least squares."""
+    m, n = A.shape
+    d = dim_checks(sampling_factor, m, n)
+    rng = np.random.default_rng(rng)
+    S = _sketch_gen(sketch, vec_nnz)(d, m, rng)
+    A_ske = S @ A
+    b_ske = S @ b
+    if delta > 0:
+        A_ske = np.vstack((A_ske, (delta ** 0.5) * np.eye(n)))
+        b_ske = np.concatenate((b_ske, np.zeros(n)))
+    return la.lstsq(A_ske, b_ske, cond=None, lapack_driver=lapack_driver)[0]
+
+
+def pcg(mv_mat, rhs, mv_pre, iter_lim, tol, x0):
+    """comps/determiter/pcg.py:5-47: preconditioned CG on mat x = rhs with
+    mv_pre = M M^T; recomputes the residual every 10 steps; residuals[i] =
+    ||rhs - mat x_i|| (the recurrence's value), stop at tol * ||r_0||."""
+    x = x0.copy()
+    residuals = -np.ones(iter_lim)
+    r = rhs - mv_mat(x)
+    d = mv_pre(r)
+    delta1_new = np.dot(r, d)
+    cur_err = la.norm(r)
+    rel_tol = tol * cur_err
+    i = 0
+    while i < iter_lim and cur_err > rel_tol:
+        residuals[i] = cur_err
+        q = mv_mat(d)
+        alpha = delta1_new / np.dot(d, q)
+        x += alpha * d
+        if i % 10 == 0:
+            r = rhs - mv_mat(x)
+        else:
+            r -= alpha * q
+        cur_err = la.norm(r)
+        s_ = mv_pre(r)
+        delta1_old = delta1_new
+        delta1_new = np.dot(r, s_)
+        d = s_ + (delta1_new / delta1_old) * d
+        i += 1
+    return x, residuals[:i]
+
+
+def pcss1(A, b, c, delta, tol, iter_lim, R, upper_tri, z0):
+    """PcSS1.__call__, comps/determiter/saddle.py:88-176: PCG on the normal
+    equations (A^T A + delta I) x = A^T b - c with the preconditioner
+    R R^T (+ I - V V^T when R has fewer columns than A, with R rescaled by
+    the regularised singular values)."""
+    m, n = A.shape
+    if b is None:
+        b = np.zeros(m)
+    if upper_tri:
+        raise NotImplementedError()
+    R = np.array(R, dtype=np.float64)
+    pc_dim = R.shape[1]
+    full = pc_dim == n
+    if not full:
+        sv = 1.0 / la.norm(R, axis=0)
+        V = R * sv
+        sv = np.sqrt(sv ** 2 + delta)
+        R = V / (sv / sv[-1])
+
+    def mv_pre(vec):
+        res = R @ (R.T @ vec)
+        if not full:
+            res = res + vec - V @ (V.T @ vec)
+        return res
+
+    def mv_gram(vec):
+        return A.T @ (A @ vec) + delta * vec
+
+    rhs = A.T @ b
+    if c is not None:
+        rhs = rhs - c
+    x0 = np.zeros(n) if (z0 is None or not full) else R @ z0
+    x, residuals = pcg(mv_gram, rhs, mv_pre, iter_lim, tol, x0)
+    return x, b - A @ x, residuals
+
+
+def pcss2_saddle(A, b, c, delta, tol, iter_lim, R, upper_tri, z0):
+    """PcSS2.__call__ dispatch, saddle.py:186-217."""
+    if c is None or la.norm(c) == 0:
+        x, y, errors, _ = pcss2(A, b, delta, tol, iter_lim, R, upper_tri, z0)
+        return x, y, errors
+    if b is None or la.norm(b) == 0:
+        x, y, errors, _ = pcss2_under(A, c, delta, tol, iter_lim, R, upper_tri)
+        return x, y, errors
+    raise ValueError('One of "b" or "c" must be zero.')
+
+
+def sps1(A, b, c, delta, tol, iter_lim, rng, solver=None, sampling_factor=3, sketch="gaussian"):
+    """SPS1.__call__, drivers/saddlesys.py:124-231 (Nystrom strategy 'left'
+    when d < n).  ``solver(A, b, c, delta, tol, iter_lim, M, upper_tri, z0)``."""
+    solver = solver or pcss1
+    m, n = A.shape
+    sqrt_delta = np.sqrt(delta)
+    d = int(sampling_factor * n)
+    rng = np.random.default_rng(rng)
+    if b is None:
+        b = np.zeros(m)
+    log = SketchAndPrecondLog()
+    gen = _sketch_gen(sketch)
+    nystrom_like = d < n
+    if not nystrom_like:
+        A_ske = gen(d, m, rng) @ A
+        if sqrt_delta > 0:
+            A_ske = np.vstack((A_ske, sqrt_delta * np.eye(n)))
+        M, U, sigma, Vh = svd_right_precond(A_ske)
+    else:
+        A_ske = gen(d, m, rng) @ A
+        V = la.qr(A_ske.T, mode="economic")[0]
+        A_sample = A @ V
+        _, sigma, Wt = la.svd(A_sample, full_matrices=False)
+        M = V @ (Wt.T / sigma)
+    rhs = A.T @ b
+    if c is not None:
+        rhs = rhs - c
+    z_ske = None
+    if not nystrom_like:
+        z_ske = (Vh @ rhs) / sigma
+        x_ske = M @ z_ske
+        rhs_pc = M.T @ rhs
+        lhs_ske_pc = M.T @ (A.T @ (A @ x_ske) + delta * x_ske)
+        if la.norm(lhs_ske_pc - rhs_pc) >= la.norm(rhs_pc):
+            z_ske = None
+    x, y, errors = solver(A, b, c, delta, tol, iter_lim, M, False, z_ske)
+    log.wrap_up(errors, la.norm(rhs))
+    return x, y, log
+
+
+def sps2(A, b, c, delta, tol, iter_lim, rng, sampling_factor=3, sketch="gaussian"):
+    """SPS2.__call__, drivers/saddlesys.py:280-327: the saddle-point system as
+    an overdetermined LS problem on [A; sqrt(delta) I] with the right-hand
+    side shifted by the sketched solution of the c-part, then PcSS2."""
+    m, n = A.shape
+    sqrt_delta = np.sqrt(delta)
+    d = dim_checks(sampling_factor, m, n)
+    rng = np.random.default_rng(rng)
+    if b is None:
+        b = np.zeros(m)
+    log = SketchAndPrecondLog()
+    S = _sketch_gen(sketch)(d, m, rng)
+    A_ske = S @ A
+    if sqrt_delta > 0:
+        A_ske = np.vstack((A_ske, sqrt_delta * np.eye(n)))
+    M, U, sigma, Vh = svd_right_precond(A_ske)
+    A_aug = np.vstack((A, sqrt_delta * np.eye(n))) if sqrt_delta > 0 else A
+    b_aug = np.concatenate((b, np.zeros(n))) if delta > 0 else b.copy()
+    if c is not None and la.norm(c) > 0:
+        v = U @ ((1 / sigma) * (Vh @ c))
+        b_aug[:m] -= S.T @ v[:d]
+        if delta > 0:
+            b_aug[m:] -= v[d:]
+    z_ske = U[:d, :].T @ (S @ b_aug[:m])
+    if delta > 0:
+        z_ske += U[d:, :].T @ b_aug[m:]
+    x_ske = M @ z_ske
+    if la.norm(b_aug - A_aug @ x_ske) >= la.norm(b_aug):
+        z_ske = None
+    x, _, errors, _ = pcss2(A_aug, b_aug, 0.0, tol, iter_lim, M, False, z_ske)
+    y = b - A @ x
+    log.wrap_up(errors, la.norm(M.T @ (A_aug.T @ b_aug)))
+    return x, y, log
+
+
+def project_out(Qi, Q):
+    """comps/qb.py:598-613 (as_list=False): Qi - Q (Q^T Qi)."""
+    return Qi - Q @ (Q.T @ Qi)
+
+
+def qb2(A, k, tol, rng, rf, blk):
+    """QB2.__call__, comps/qb.py:437-482 (overwrite_a=False): blocks of
+    rangefinder columns projected out of the previous ones, A deflated by
+    each block; stops at k columns or when ||A - Q B||_F <= tol ||A||_F.
+    The reference passes the removed ``np.NaN`` as the rangefinder's tol
+    (:468); NaN is meant."""
+    A = np.array(A, dtype=np.float64)
+    assert k > 0
+    k = min(k, min(A.shape))
+    use_tol = not np.isnan(tol) and tol > 0
+    if use_tol:
+        sq_norm_A = la.norm(A, "fro") ** 2
+        abs_sq_tol = sq_norm_A * tol ** 2
+    rng = np.random.default_rng(rng)
+    Q = np.empty((A.shape[0], 0))
+    B = np.empty((0, A.shape[1]))
+    while True:
+        if B.shape[0] + blk > k:
+            blk = k - B.shape[0]
+        Qi = rf(A, blk, np.nan, rng)
+        Qi = project_out(Qi, Q)
+        Qi = la.qr(Qi, mode="economic")[0]
+        Bi = Qi.T @ A
+        Q = np.column_stack((Q, Qi))
+        B = np.vstack((B, Bi))
+        A -= Qi @ Bi
+        if use_tol:
+            sq_norm_A = sq_norm_A - la.norm(Bi, "fro") ** 2
+            if sq_norm_A <= abs_sq_tol:
+                break
+        if B.shape[0] >= k:
+            break
+    return Q, B
+
+
+def qb3(A, k, tol, rng, sk_op, blk):
+    """QB3.__call__, comps/qb.py:552-595: G = A S, H = A^T G once, then blocks
+    of Q and B from them without further passes over A (early stopping on
+    the Frobenius-norm estimate).  ``sk_op(A, k, rng)``."""
+    assert 0 < k < min(A.shape)
+    use_tol = not np.isnan(tol) and tol > 0
+    if use_tol:
+        sq_norm_A = la.norm(A, "fro") ** 2
+        abs_sq_tol = sq_norm_A * tol ** 2
+    Q = np.empty((A.shape[0], 0))
+    B = np.empty((0, A.shape[1]))
+    rng = np.random.default_rng(rng)
+    S = sk_op(A, k, rng)
+    G = A @ S
+    H = A.T @ G
+    for i in range(int(np.ceil(k / blk))):
+        b0, b1 = i * blk, min((i + 1) * blk, k)
+        Si = S[:, b0:b1]
+        BSi = B @ Si
+        Yi = G[:, b0:b1] - Q @ BSi
+        Qi, Ri = la.qr(Yi, mode="economic")
+        Qi = project_out(Qi, Q)
+        Qi, Rihat = la.qr(Qi, mode="economic")
+        Ri = Rihat @ Ri
+        Bi = H[:, b0:b1].T - (Yi.T @ Q) @ B - BSi.T @ B
+        Bi = la.solve_triangular(Ri, Bi, trans="T")
+        Q = np.column_stack((Q, Qi))
+        B = np.vstack((B, Bi))
+        if use_tol:
+            sq_norm_A = sq_norm_A - la.norm(Bi, "fro") ** 2
+            if sq_norm_A <= abs_sq_tol:
+                break
+    return Q, B
+
+
+def evd2(A, k, tol, over, rng, sk_op):
+    """EVD2.__call__, drivers/evd.py:349-384: Nystrom approximation of a PSD
+    A from S = sk_op(A, k + over, rng), regularised by nu = sqrt(n) eps ||A S||,
+    truncated where sigma_{i+1}^2 <= nu."""
+    n = A.shape[0]
+    assert 0 < k < n
+    rng = np.random.default_rng(rng)
+    S = sk_op(A, k + over, rng)
+    Y = A @ S
+    nu = np.sqrt(n) * np.finfo(float).eps * la.norm(Y)
+    Y = Y + nu * S
+    R = la.cholesky(S.T @ Y, lower=False)
+    B = la.solve_triangular(R, Y.T, lower=False, trans=True).T
+    V, sigma, _ = la.svd(B)
+    comp = [k]
+    for i in range(k - 1):
+        if sigma[i + 1] ** 2 <= nu:
+            comp.append(i)
+    r = min(comp)
+    return V[:, :r], (sigma ** 2)[:r] - nu

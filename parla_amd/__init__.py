@@ -31,6 +31,21 @@ def __getattr__(name):
         'PcSS2': 'parla_amd.comps.determiter.saddle', 'lsqr': 'parla_amd.comps.determiter.lsqr',
         'a_lift_precond': 'parla_amd.comps.preconditioning',
         'orth': 'parla_amd.utils.linalg_wrappers', 'lu_stabilize': 'parla_amd.utils.linalg_wrappers',
+        'SkOpSS': 'parla_amd.comps.sketchers.oblivious', 'SkOpTC': 'parla_amd.comps.sketchers.oblivious',
+        'srct_operator': 'parla_amd.utils.sketching', 'sparse_sign_operator': 'parla_amd.utils.sketching',
+        'apply_srct': 'parla_amd.utils.sketching', 'generate_srct': 'parla_amd.utils.sketching',
+        'SSO1': 'parla_amd.drivers.least_squares', 'sso1': 'parla_amd.drivers.least_squares',
+        'spo1': 'parla_amd.drivers.least_squares', 'spo3': 'parla_amd.drivers.least_squares',
+        'SPU1': 'parla_amd.drivers.least_squares', 'spu1': 'parla_amd.drivers.least_squares',
+        'UnderLstsqSolver': 'parla_amd.drivers.least_squares',
+        'svd_right_precond': 'parla_amd.comps.preconditioning', 'a_lift': 'parla_amd.comps.preconditioning',
+        'PrecondSaddleSolver': 'parla_amd.comps.determiter.saddle',
+        'QB2': 'parla_amd.comps.qb', 'QB3': 'parla_amd.comps.qb', 'qb': 'parla_amd.comps.qb',
+        'EVD2': 'parla_amd.drivers.evd', 'evd1': 'parla_amd.drivers.evd', 'evd2': 'parla_amd.drivers.evd',
+        'svd1': 'parla_amd.drivers.svd',
+        'PcSS1': 'parla_amd.comps.determiter.saddle', 'pcg': 'parla_amd.comps.determiter.pcg',
+        'SPS1': 'parla_amd.drivers.saddlesys', 'SPS2': 'parla_amd.drivers.saddlesys',
+        'SaddleSolver': 'parla_amd.drivers.saddlesys', 'sps': 'parla_amd.drivers.saddlesys',
     }
     if name in table:
         return getattr(importlib.import_module(table[name]), name)

@@ -48,6 +48,7 @@ SIGNATURES = {
     "pamd_trsv_upper_f64": (c_int, [c_int, c_i64, c_vp, c_i64, c_vp, c_vp]),
     "pamd_gaussian_fill_f64": (c_int, [c_i64, c_i64, c_dbl, c_u64, c_u64, c_vp, c_i64, c_vp]),
     "pamd_sjlt_draw": (c_int, [c_i64, c_i64, c_int, c_u64, c_vp, c_vp, c_vp]),
+    "pamd_srct_fill_f64": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "pamd_sjlt_csr_workspace": (c_int, [c_i64, c_i64, ctypes.POINTER(c_i64)]),
     "pamd_sjlt_csr": (c_int, [c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_sjlt_left_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp,
@@ -55,6 +56,7 @@ SIGNATURES = {
     "pamd_sjlt_right_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp,
                                     c_i64, c_vp]),
     "pamd_transpose_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
+    "pamd_scale_cols_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_int, c_vp, c_i64, c_vp]),
     "pamd_geqrf_workspace": (c_int, [c_i64, c_i64, c_i64, ctypes.POINTER(c_i64)]),
     "pamd_geqrf_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp]),
     "pamd_orgqr_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp]),
@@ -109,6 +111,7 @@ SIGNATURES = {
     "pamd_mlp_adam": (c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                               ctypes.c_float, c_i64, c_vp, c_vp, c_vp]),
     "pamd_row_gather_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp]),
+    "pamd_row_scatter_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp]),
     "pamd_lsqr_workspace": (c_int, [c_i64, c_i64, c_i64, ctypes.POINTER(c_i64)]),
     "pamd_lsqr_run_f64": (c_int, [ctypes.POINTER(LsqrProblem), c_vp, c_vp, c_vp, c_vp,
                                   ctypes.POINTER(LsqrResult), c_vp, c_vp]),

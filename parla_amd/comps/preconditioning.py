@@ -66,3 +66,12 @@ def a_lift_precond(A, delta, R, upper_tri=False, k=1):
         raise NotImplementedError()
     op = PrecondOperator(A, delta, R, upper_tri)
     return op, op.m_fwd, op.m_adj
+
+
+def svd_right_precond(A_ske):
+    """preconditioning.py:70-79: (M, U, sigma, Vh) with M = V / sigma over the
+    numerical rank of the sketch (sigma > sigma_1 * ncols * eps), computed on
+    the device; results have the kind of the input."""
+    t = ops.as_device(A_ske)
+    M, U, sig, Vh = ops.svd_right_precond(t)
+    return tuple(ops.like_input(v, A_ske) for v in (M, U, sig, Vh))

@@ -44,6 +44,24 @@ class SkOpSJ(SketchOpGen):
         return usk.sjlt_operator(n_rows, n_cols, rng, self.vec_nnz, backend=self.backend)
 
 
+class SkOpSS(SketchOpGen):
+    """Sparse sign operator, oblivious.py:58-65 (the reference's __call__
+    drops its result; this one returns it)."""
+
+    def __init__(self, density=0.05):
+        self.density = density
+
+    def __call__(self, n_rows, n_cols, rng):
+        return usk.sparse_sign_operator(n_rows, n_cols, rng, self.density)
+
+
+class SkOpTC(SketchOpGen):
+    """SRTT based on the DCT-II (the SRCT), oblivious.py:68-72."""
+
+    def __call__(self, n_rows, n_cols, rng):
+        return usk.srct_operator(n_rows, n_cols, rng)
+
+
 class SkOpIN(SketchOpGen):
     """Row / column selection, oblivious.py:75-82."""
 

@@ -339,6 +339,15 @@ __global__ __launch_bounds__(1024) void trsv_kernel(int trans, int n, const doub
   for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = xs[i];
 }
 
+// n beyond what LDS holds: the solution vector stays in global memory (one
+// workgroup, so its stores are visible to its own waves after each barrier);
+// only the 64 x 64 diagonal block and the reductions use LDS.
+__global__ __launch_bounds__(1024) void trsv_global_kernel(int trans, int n, const double* __restrict__ R,
+                                                           int64_t ldr, double* x) {
+  extern __shared__ double smem[];
+  trsv_upper_block(trans, n, R, ldr, x, smem);
+}
+
 }  // namespace
 
 extern "C" int pamd_gemm_workspace(int64_t M, int64_t N, int64_t K, int64_t* bytes) {
@@ -472,13 +481,18 @@ extern "C" int pamd_nrm2_large_f64(int64_t n, const double* x, double* result, v
 
 extern "C" int pamd_trsv_upper_f64(int trans, int64_t n, const double* R, int64_t ldr, double* x, void* stream) {
   PAMD_ARG_CHECK(trans == 0 || trans == 1, 1);
-  PAMD_ARG_CHECK(n >= 0 && n <= PAMD_TRSV_MAX_N, 2);
+  PAMD_ARG_CHECK(n >= 0 && n < (int64_t)1 << 30, 2);
   PAMD_ARG_CHECK(R != nullptr || n == 0, 3);
   PAMD_ARG_CHECK(ldr >= n && ldr >= 1, 4);
   PAMD_ARG_CHECK(x != nullptr || n == 0, 5);
   if (n == 0) return PAMD_OK;
-  const size_t shmem = ((size_t)n + TRSV_SCRATCH_DOUBLES) * sizeof(double);
-  trsv_kernel<<<1, 1024, shmem, (hipStream_t)stream>>>(trans, (int)n, R, ldr, x);
+  if (n <= PAMD_TRSV_MAX_N) {
+    const size_t shmem = ((size_t)n + TRSV_SCRATCH_DOUBLES) * sizeof(double);
+    trsv_kernel<<<1, 1024, shmem, (hipStream_t)stream>>>(trans, (int)n, R, ldr, x);
+  } else {
+    trsv_global_kernel<<<1, 1024, TRSV_SCRATCH_DOUBLES * sizeof(double), (hipStream_t)stream>>>(trans, (int)n, R,
+                                                                                               ldr, x);
+  }
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

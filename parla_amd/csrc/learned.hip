@@ -255,6 +255,15 @@ __global__ void row_gather_kernel(int64_t d, int64_t n, const int64_t* __restric
   for (int64_t c = threadIdx.x; c < n; c += blockDim.x) out[r * ldo + c] = sc * src[c];
 }
 
+// out[idx[r], :] = A[r, :]: one workgroup per source row
+__global__ void row_scatter_kernel(int64_t n, const int64_t* __restrict__ idx, const double* __restrict__ A,
+                                   int64_t lda, double* __restrict__ out, int64_t ldo) {
+  const int64_t r = blockIdx.x;
+  double* dst = out + idx[r] * ldo;
+  const double* src = A + r * lda;
+  for (int64_t c = threadIdx.x; c < n; c += blockDim.x) dst[c] = src[c];
+}
+
 }  // namespace
 
 extern "C" int pamd_mlp_nparams(void) { return MLP_NPARAM; }
@@ -332,6 +341,22 @@ extern "C" int pamd_row_gather_f64(int64_t d, int64_t m, int64_t n, const int64_
   PAMD_ARG_CHECK(ldo >= n, 9);
   if (d == 0 || n == 0) return PAMD_OK;
   row_gather_kernel<<<(unsigned)d, 256, 0, (hipStream_t)stream>>>(d, n, idx, scale, A, lda, out, ldo);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_row_scatter_f64(int64_t d, int64_t m, int64_t n, const int64_t* idx, const double* A,
+                                    int64_t lda, double* out, int64_t ldo, void* stream) {
+  PAMD_ARG_CHECK(d >= 0, 1);
+  PAMD_ARG_CHECK(m >= d, 2);
+  PAMD_ARG_CHECK(n >= 0, 3);
+  PAMD_ARG_CHECK(idx != nullptr || d == 0, 4);
+  PAMD_ARG_CHECK(A != nullptr || d == 0, 5);
+  PAMD_ARG_CHECK(lda >= n, 6);
+  PAMD_ARG_CHECK(out != nullptr || d == 0, 7);
+  PAMD_ARG_CHECK(ldo >= n, 8);
+  if (d == 0 || n == 0) return PAMD_OK;
+  row_scatter_kernel<<<(unsigned)d, 256, 0, (hipStream_t)stream>>>(n, idx, A, lda, out, ldo);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

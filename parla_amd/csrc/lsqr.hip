@@ -39,14 +39,19 @@ struct LsqrConst {
   int64_t m, n, lda, ldr, iter_lim;
   double sd, damp, atol, btol, ctol;
   int has_r, calc_var, has_x0, grid;
+  int xs_global;  // the substitution vector lives in the workspace (n too large for LDS)
 };
+
+// largest n whose substitution vector the tail keeps in LDS next to the solve
+// scratch (<= 64 KiB of dynamic LDS)
+constexpr int64_t LSQR_LDS_MAX_N = 1900;
 
 constexpr int PASS_THREADS = 256;
 
 __host__ __device__ inline int lsqr_grid(int64_t m) { return (int)imax64(1, imin64(1024, pamd_cdiv(m, 64))); }
 
 struct LsqrLayout {
-  int64_t st, u, v, w, work, y, pv, pb, total;
+  int64_t st, u, v, w, work, y, pv, pb, xs, total;
 };
 
 __host__ __device__ inline LsqrLayout lsqr_layout(int64_t m, int64_t n) {
@@ -60,7 +65,8 @@ __host__ __device__ inline LsqrLayout lsqr_layout(int64_t m, int64_t n) {
   L.y = L.work + n;
   L.pv = (L.y + n + 31) / 32 * 32;
   L.pb = L.pv + (int64_t)G * n;
-  L.total = L.pb + (int64_t)G * 2 + 32;
+  L.xs = (L.pb + (int64_t)G * 2 + 31) / 32 * 32;
+  L.total = L.xs + (n > LSQR_LDS_MAX_N ? n : 0) + 32;
   return L;
 }
 
@@ -322,7 +328,8 @@ __global__ __launch_bounds__(1024) void lsqr_init_kernel(LsqrConst K, const doub
   }
   __syncthreads();
   if (K.has_x0) {
-    apply_forward(K, R, x, ws + L.work, smem, smem + K.n);
+    double* xs = K.xs_global ? ws + L.xs : smem;
+    apply_forward(K, R, x, ws + L.work, xs, K.xs_global ? smem : smem + K.n);
   } else {
     for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) ws[L.work + i] = 0.0;
   }
@@ -344,9 +351,9 @@ __global__ __launch_bounds__(1024) void lsqr_tail_kernel(LsqrConst K, const doub
   __shared__ double red[16];
   __shared__ double cred[16][64];
   __shared__ double bc[2];
-  double* xs = smem;
-  double* scratch = smem + K.n;
   const LsqrLayout L = lsqr_layout(K.m, K.n);
+  double* xs = K.xs_global ? ws + L.xs : smem;
+  double* scratch = K.xs_global ? smem : smem + K.n;
   double* v = ws + L.v;
   double* w = ws + L.w;
   const int64_t n = K.n;
@@ -502,9 +509,12 @@ __global__ __launch_bounds__(1024) void lsqr_tail_kernel(LsqrConst K, const doub
 
 // x_orig = R^{-1} x (the M_fwd of comps/determiter/saddle.py:198)
 __global__ __launch_bounds__(1024) void lsqr_xorig_kernel(LsqrConst K, const double* __restrict__ R,
-                                                          const double* __restrict__ x, double* __restrict__ xo) {
+                                                          const double* __restrict__ x, double* __restrict__ xo,
+                                                          double* __restrict__ ws) {
   extern __shared__ double smem[];
-  apply_forward(K, R, x, xo, smem, smem + K.n);
+  const LsqrLayout L = lsqr_layout(K.m, K.n);
+  double* xs = K.xs_global ? ws + L.xs : smem;
+  apply_forward(K, R, x, xo, xs, K.xs_global ? smem : smem + K.n);
 }
 
 int launch_pass(const LsqrConst& K, const double* A, double* ws, hipStream_t st) {
@@ -543,8 +553,9 @@ extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, doubl
   PAMD_ARG_CHECK(P.m >= 0 && P.n >= 1, 1);
   PAMD_ARG_CHECK(P.A != nullptr && P.lda >= P.n, 1);
   PAMD_ARG_CHECK(P.b != nullptr, 1);
-  // with R the tail keeps x, the solve scratch and its own reduction space in LDS (<= 64 KiB)
-  PAMD_ARG_CHECK(P.R == nullptr || (P.ldr >= P.n && P.n <= 1900), 1);
+  // with R the tail keeps the substitution vector in LDS up to LSQR_LDS_MAX_N, in the workspace beyond
+  PAMD_ARG_CHECK(P.R == nullptr || P.ldr >= P.n, 1);
+  PAMD_ARG_CHECK(P.n < ((int64_t)1 << 30), 1);
   PAMD_ARG_CHECK(x != nullptr, 2);
   PAMD_ARG_CHECK(var != nullptr || !P.calc_var, 4);
   PAMD_ARG_CHECK(arnorms != nullptr || P.iter_lim <= 0, 5);
@@ -566,8 +577,10 @@ extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, doubl
   K.has_x0 = P.x0 != nullptr;
   K.grid = lsqr_grid(P.m);
   K.ldr = P.ldr;
+  K.xs_global = (K.has_r && P.n > LSQR_LDS_MAX_N) ? 1 : 0;
   double* ws = (double*)workspace;
-  const size_t shmem = (size_t)(K.has_r ? (P.n + TRSV_SCRATCH_DOUBLES) : 1) * sizeof(double);
+  const size_t shmem =
+      (size_t)(K.has_r ? ((K.xs_global ? 0 : P.n) + TRSV_SCRATCH_DOUBLES) : 1) * sizeof(double);
 
   lsqr_init_kernel<<<1, 1024, shmem, st>>>(K, P.R, P.b, P.x0, x, var, ws);
   PAMD_LAUNCH_CHECK();
@@ -594,7 +607,7 @@ extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, doubl
     batch = imin64(batch * 2, 64);
   }
   if (x_orig) {
-    lsqr_xorig_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, x_orig);
+    lsqr_xorig_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, x_orig, ws);
     PAMD_LAUNCH_CHECK();
   }
   result->istop = hs.istop;

@@ -341,3 +341,72 @@ extern "C" int pamd_transpose_f64(int64_t rows, int64_t cols, const double* A, i
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }
+
+namespace {
+
+// out[i][j] = A[i][j] * s[j] (invert = 0) or A[i][j] / s[j] (invert = 1)
+__global__ void scale_cols_kernel(int64_t rows, int64_t cols, const double* __restrict__ A, int64_t lda,
+                                  const double* __restrict__ s, int invert, double* __restrict__ out, int64_t ldo) {
+  const int64_t i = blockIdx.y;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < cols; j += (int64_t)gridDim.x * blockDim.x) {
+    const double a = A[i * lda + j];
+    out[i * ldo + j] = invert ? a / s[j] : a * s[j];
+  }
+}
+
+}  // namespace
+
+extern "C" int pamd_scale_cols_f64(int64_t rows, int64_t cols, const double* A, int64_t lda, const double* s,
+                                   int invert, double* out, int64_t ldo, void* stream) {
+  PAMD_ARG_CHECK(rows >= 0 && rows < (int64_t)1 << 31, 1);
+  PAMD_ARG_CHECK(cols >= 0, 2);
+  PAMD_ARG_CHECK(A != nullptr || rows * cols == 0, 3);
+  PAMD_ARG_CHECK(lda >= cols && lda >= 1, 4);
+  PAMD_ARG_CHECK(s != nullptr || cols == 0, 5);
+  PAMD_ARG_CHECK(invert == 0 || invert == 1, 6);
+  PAMD_ARG_CHECK(out != nullptr || rows * cols == 0, 7);
+  PAMD_ARG_CHECK(ldo >= cols && ldo >= 1, 8);
+  if (rows == 0 || cols == 0) return PAMD_OK;
+  dim3 grid((unsigned)imin64(pamd_cdiv(cols, 256), 64), (unsigned)rows);
+  scale_cols_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(rows, cols, A, lda, s, invert, out, ldo);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+namespace {
+
+// Rows r_i of the orthonormal DCT-II of length N applied after the sign flip
+// and permutation of the SRCT: out[i][j] = s(r_i) cos(pi r_i (2 q + 1) / (2N)) e[q]
+// with q = pinv[j], s(0) = sqrt(1/N), s(k) = sqrt(2/N).  The argument is
+// reduced exactly in integers (period 4N) before the cosine.
+__global__ void srct_fill_kernel(int64_t N, const int64_t* __restrict__ r, const double* __restrict__ e,
+                                 const int64_t* __restrict__ pinv, double* __restrict__ out, int64_t ld) {
+  const int64_t i = blockIdx.y;
+  const int64_t ri = r[i];
+  const double si = (ri == 0) ? sqrt(1.0 / (double)N) : sqrt(2.0 / (double)N);
+  const int64_t period = 4 * N;
+  const double inv2n = 1.0 / (double)(2 * N);
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = pinv[j];
+    const int64_t t = (ri * (2 * q + 1)) % period;
+    out[i * ld + j] = si * cospi((double)t * inv2n) * e[q];
+  }
+}
+
+}  // namespace
+
+extern "C" int pamd_srct_fill_f64(int64_t d, int64_t N, const int64_t* r, const double* e, const int64_t* pinv,
+                                  double* out, int64_t ld, void* stream) {
+  PAMD_ARG_CHECK(d >= 0 && d <= N && d < (int64_t)1 << 31, 1);
+  PAMD_ARG_CHECK(N >= 0 && N < (int64_t)1 << 30, 2);
+  PAMD_ARG_CHECK(r != nullptr || d == 0, 3);
+  PAMD_ARG_CHECK(e != nullptr || N == 0, 4);
+  PAMD_ARG_CHECK(pinv != nullptr || N == 0, 5);
+  PAMD_ARG_CHECK(out != nullptr || d * N == 0, 6);
+  PAMD_ARG_CHECK(ld >= N && ld >= 1, 7);
+  if (d == 0 || N == 0) return PAMD_OK;
+  dim3 grid((unsigned)imin64(pamd_cdiv(N, 256), 64), (unsigned)d);
+  srct_fill_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(N, r, e, pinv, out, ld);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}

@@ -1,7 +1,8 @@
 // Block-level upper-triangular solve used by the standalone TRSV entry point
 // and by the LSQR iteration kernel.
 //
-// The solution vector lives in LDS.  The triangle is processed in 64-row
+// The solution vector lives in LDS (global memory for n beyond what LDS
+// holds).  The triangle is processed in 64-row
 // blocks: the off-diagonal part of each block is a GEMV spread over all the
 // block's wavefronts (row-wise for R x = b, column-wise for R^T x = b, both
 // reading contiguous 512-byte row segments of the row-major R), then the
@@ -17,7 +18,9 @@ constexpr int TRSV_SCRATCH_DOUBLES = 64 * 65 + 16 * 64 + 64;
 
 // All threads of the block (blockDim.x a multiple of 64, at most 1024) must
 // call this.  trans = 0 solves R x = b, trans = 1 solves R^T x = b; xs holds b
-// on entry and x on exit.  scratch: TRSV_SCRATCH_DOUBLES doubles of LDS.
+// on entry and x on exit, in LDS or (for n beyond what LDS holds) in global
+// memory that only this workgroup touches.  scratch: TRSV_SCRATCH_DOUBLES
+// doubles of LDS.
 __device__ inline void trsv_upper_block(int trans, int n, const double* __restrict__ R, int64_t ldr,
                                         double* xs, double* scratch) {
   double(*Rd)[65] = reinterpret_cast<double(*)[65]>(scratch);

@@ -16,25 +16,20 @@ class SVDecomposer:
 
 
 def small_svd(B):
-    """Thin SVD of a short-wide (or small) device matrix B (kb x n).
+    """Thin SVD of the k x n factor B on the device (ops.thin_svd: Householder
+    QR of the taller orientation, then the k x k SVD of its triangle)."""
+    return ops.thin_svd(B)
 
-    B^T = Q2 R2 by the library's Householder QR, so B = R2^T Q2^T; the kb x kb
-    SVD R2^T = Ur diag(s) Vr^T (rocSOLVER's dgesdd) gives U = Ur,
-    Vh = (Q2 Vr)^T.  The large factor never leaves the device.
-    """
-    kb, n = B.shape
-    if kb <= n:
-        f = ops.geqrf_(ops.transpose(B))
-        Q2 = f.Q()
-        R2 = torch.triu(f.R_view)
-        Ur, s, Vrh = ops.svd_small(R2.t())
-        Vh = ops.transpose(ops.gemm(Q2, Vrh.t()))
-        return Ur.contiguous(), s, Vh
-    f = ops.geqrf_(B.contiguous().clone())
-    Q3 = f.Q()
-    R3 = torch.triu(f.R_view)
-    Ur, s, Vh = torch.linalg.svd(R3, full_matrices=False)
-    return ops.gemm(Q3, Ur), s, Vh.contiguous()
+
+def svd1(A, k, over, tol, inner_num_pass, block_size, rng):
+    """svd.py:58-123: SVD1 on QB2(RF1(RS1(Gaussian, inner_num_pass - 2)), block_size)."""
+    from parla_amd.comps.qb import QB2, RF1, RS1
+    from parla_amd.comps.sketchers.oblivious import SkOpGA
+    import parla_amd.utils.linalg_wrappers as ulaw
+    rng = np.random.default_rng(rng)
+    rso_ = RS1(SkOpGA(), inner_num_pass - 2, ulaw.orth, 1)
+    qb_ = QB2(RF1(rso_), block_size, overwrite_a=False)
+    return SVD1(qb_)(A, k, tol, over, rng)
 
 
 class SVD1(SVDecomposer):

@@ -184,6 +184,29 @@ def trsv_(R, x, trans=False):
     return x
 
 
+def row_gather(A, idx, scale=None, out=None):
+    """out[r, :] = scale[r] * A[idx[r], :] (scale None: 1); idx a device int64 tensor."""
+    A = A if A.stride(-1) == 1 else A.contiguous()
+    m, n = A.shape
+    d = idx.numel()
+    if out is None:
+        out = torch.empty((d, n), dtype=F64, device=A.device)
+    check(lib().pamd_row_gather_f64(d, m, n, _p(idx), _p(scale), _p(A), max(A.stride(0), n, 1), _p(out),
+                                    max(out.stride(0), n, 1), stream(A.device)), "pamd_row_gather_f64")
+    return out
+
+
+def row_scatter(A, idx, rows, out=None):
+    """out (rows x n, zero outside the selected rows) with out[idx[r], :] = A[r, :]."""
+    A = A if A.stride(-1) == 1 else A.contiguous()
+    d, n = A.shape
+    if out is None:
+        out = torch.zeros((rows, n), dtype=F64, device=A.device)
+    check(lib().pamd_row_scatter_f64(d, rows, n, _p(idx), _p(A), max(A.stride(0), n, 1), _p(out),
+                                     max(out.stride(0), n, 1), stream(A.device)), "pamd_row_scatter_f64")
+    return out
+
+
 def transpose(A):
     """Contiguous A^T."""
     A = A.contiguous()
@@ -381,6 +404,15 @@ def _rocsolver():
     if _ROCSOLVER is None:
         rb = ctypes.CDLL("librocblas.so")
         rs = ctypes.CDLL("librocsolver.so")
+        vp, i32 = ctypes.c_void_p, ctypes.c_int
+        rb.rocblas_create_handle.restype = i32
+        rb.rocblas_create_handle.argtypes = [ctypes.POINTER(vp)]
+        rb.rocblas_set_stream.restype = i32
+        rb.rocblas_set_stream.argtypes = [vp, vp]
+        # rocblas_status rocsolver_dgesdd(handle, left_svect, right_svect, m, n, A, lda, S,
+        #                                 U, ldu, V, ldv, info)
+        rs.rocsolver_dgesdd.restype = i32
+        rs.rocsolver_dgesdd.argtypes = [vp, i32, i32, i32, i32, vp, i32, vp, vp, i32, vp, i32, vp]
         h = ctypes.c_void_p()
         if rb.rocblas_create_handle(ctypes.byref(h)) != 0:
             raise RuntimeError("rocblas_create_handle failed")
@@ -408,7 +440,86 @@ def svd_small(M):
     st = rs.rocsolver_dgesdd(h, 192, 192, n, n, _p(A), n, _p(s), _p(Ub), n, _p(Vb), n, _p(info))
     if st != 0:
         raise RuntimeError(f"rocsolver_dgesdd failed (status {st})")
+    bad = int(info.item())  # scipy's la.svd raises LinAlgError when gesdd does not converge
+    if bad != 0:
+        raise np.linalg.LinAlgError(f"SVD did not converge (rocsolver_dgesdd info {bad})")
     return Vb, s, Ub
+
+
+def thin_svd(X):
+    """X = U diag(s) Vh (thin: k = min(d, n) singular triplets) of a device
+    matrix X (d x n).  Householder QR of the taller orientation, then the k x k
+    SVD of its triangular factor (svd_small): for d >= n, X = Q R and
+    R = Ur diag(s) Vh give U = Q Ur; for d < n, X^T = Q R gives X = R^T Q^T,
+    R^T = Ur diag(s) Vrh and Vh = (Q Vrh^T)^T.  The large factor stays on the
+    device.  Returns (U, s, Vh)."""
+    d, n = X.shape
+    if d >= n:
+        f = geqrf_(X.contiguous().clone())
+        Ur, s, Vh = svd_small(torch.triu(f.R_view))
+        return gemm(f.Q(), Ur), s, Vh.contiguous()
+    f = geqrf_(transpose(X))
+    Ur, s, Vrh = svd_small(torch.triu(f.R_view).t())
+    return Ur.contiguous(), s, transpose(gemm(f.Q(), Vrh.t()))
+
+
+def svd_right_precond(A_ske):
+    """comps/preconditioning.py:70-79 on the device: the thin SVD of the
+    sketch, its numerical rank (sigma > sigma_1 * ncols * eps, one host read)
+    and M = V / sigma (n x rank).  Returns (M, U, sigma, Vh) truncated to the
+    rank."""
+    U, s, Vh = thin_svd(A_ske)
+    sh = s.cpu().numpy()
+    eps = np.finfo(float).eps
+    rank = int(np.count_nonzero(sh > sh[0] * A_ske.shape[1] * eps)) if sh.size else 0
+    U, s, Vh = U[:, :rank], s[:rank].contiguous(), Vh[:rank, :]
+    M = transpose(Vh)
+    scale_cols(M, s, invert=True, out=M)
+    return M, U, s, Vh
+
+
+def scale_cols(A, s, invert=False, out=None):
+    """out = A diag(s) (or A diag(s)^{-1}); A row-major (unit column stride)."""
+    if A.stride(-1) != 1:
+        A = A.contiguous()
+    r, c = A.shape
+    if out is None:
+        out = torch.empty((r, c), dtype=F64, device=A.device)
+    check(lib().pamd_scale_cols_f64(r, c, _p(A), max(A.stride(0), c, 1), _p(s.contiguous()), int(bool(invert)),
+                                    _p(out), max(out.stride(0), c, 1), stream(A.device)), "pamd_scale_cols_f64")
+    return out
+
+
+POTRF_BLOCK = 256
+
+
+def cholesky_upper(G):
+    """R with R^T R = G (upper triangle of the result; scipy.linalg.cholesky
+    semantics, only G's upper triangle is read).  One potrf launch up to
+    n = 512; beyond, right-looking blocked over 256-column panels: potrf of
+    the diagonal block, R12 from R12^T = G12^T R11^{-1} (the MFMA strip
+    solve) and the trailing G22 -= R12^T R12 (MFMA GEMM).  Raises
+    numpy.linalg.LinAlgError when G is not positive definite (as la.cholesky)."""
+    n = G.shape[0]
+    R = G.contiguous().clone()
+    info = torch.zeros(1, dtype=torch.int32, device=G.device)
+    if n <= CHOLQR_MAX_K:
+        potrf_(R, info)
+        bad = int(info.item())
+        if bad:
+            raise np.linalg.LinAlgError(f"{bad}-th leading minor not positive definite")
+        return R
+    for j0 in range(0, n, POTRF_BLOCK):
+        j1 = min(n, j0 + POTRF_BLOCK)
+        potrf_(R[j0:j1, j0:j1], info)
+        bad = int(info.item())
+        if bad:
+            raise np.linalg.LinAlgError(f"{j0 + bad}-th leading minor not positive definite")
+        if j1 < n:
+            X = trsm_mfma(R[j0:j1, j1:].t(), R[j0:j1, j0:j1])   # R12^T = G12^T R11^{-1}
+            R[j0:j1, j1:] = transpose(X)
+            gemm(X, R[j0:j1, j1:], out=R[j1:, j1:], alpha=-1.0, beta=1.0)
+    return R
 
 
 def lu_pl(A):

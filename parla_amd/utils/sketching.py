@@ -176,41 +176,133 @@ class SJLTSketch:
 
 
 class SamplingSketch:
-    """Row / column selection operator (utils/sketching.py:204-232)."""
+    """Row / column selection operator (utils/sketching.py:204-232).
+
+    Wide (n_cols >= n_rows): S @ X = X[indices] and its adjoint S.T @ Y puts
+    the rows of Y at `indices` (zeros elsewhere).  The tall operator is the
+    transpose of the wide one with the same indices -- the reference's tall
+    branch hands `indices` to the generator as a seed (sketching.py:230) and
+    its rmatmat builds no result (:222-224); both are the evident intent here.
+    Every product is the library's row gather or row scatter (on the
+    transposed operand for products from the right).
+    """
 
     __array_ufunc__ = None
 
-    def __init__(self, n_rows, n_cols, indices):
+    def __init__(self, n_rows, n_cols, indices, _dev_idx=None):
         self.shape = (n_rows, n_cols)
         self.wide = n_cols >= n_rows
-        self.indices = indices  # host int64 array, sorted
+        self.indices = indices  # host int64 array
+        self._dev = _dev_idx if _dev_idx is not None else {}
 
     @property
     def T(self):
-        return SamplingSketch(self.shape[1], self.shape[0], self.indices)
+        return SamplingSketch(self.shape[1], self.shape[0], self.indices, self._dev)
 
     def _idx(self, dev):
-        return torch.as_tensor(self.indices, dtype=torch.int64, device=dev)
+        key = str(dev)
+        if key not in self._dev:
+            self._dev[key] = torch.as_tensor(np.asarray(self.indices, dtype=np.int64), device=dev)
+        return self._dev[key]
+
+    def _select(self, Xt):   # X[indices] (rows)
+        return ops.row_gather(Xt, self._idx(Xt.device))
+
+    def _spread(self, Xt, rows):   # rows of X placed at indices of a zero matrix
+        return ops.row_scatter(Xt, self._idx(Xt.device), rows)
 
     def __matmul__(self, X):
+        if isinstance(X, (DenseSketch, SJLTSketch, SamplingSketch)):
+            return NotImplemented
         Xt, vec, ref = _prep(X)
-        if self.wide:  # S @ X = X[indices]
-            out = Xt.index_select(0, self._idx(Xt.device))
-        else:  # S (tall) @ X scatters X into the selected rows
-            out = torch.zeros((self.shape[0], Xt.shape[1]), dtype=Xt.dtype, device=Xt.device)
-            out.index_copy_(0, self._idx(Xt.device), Xt)
+        if Xt.shape[0] != self.shape[1]:
+            raise ValueError(f"shape mismatch {self.shape} @ {tuple(Xt.shape)}")
+        out = self._select(Xt) if self.wide else self._spread(Xt, self.shape[0])
         return _finish(out, vec, ref)
 
     def __rmatmul__(self, X):
         Xt, vec, ref = _prep(X)
         if vec:
             return _finish((self.T @ Xt), True, ref)
-        if self.wide:  # X @ S scatters columns
-            out = torch.zeros((Xt.shape[0], self.shape[1]), dtype=Xt.dtype, device=Xt.device)
-            out.index_copy_(1, self._idx(Xt.device), Xt)
-        else:  # X @ S_tall = X[:, indices]
-            out = Xt.index_select(1, self._idx(Xt.device))
-        return _finish(out, False, ref)
+        if Xt.shape[1] != self.shape[0]:
+            raise ValueError(f"shape mismatch {tuple(Xt.shape)} @ {self.shape}")
+        # X @ S = (S^T X^T)^T
+        base, trans, _ = ops.mat_desc(Xt)
+        XT = base if trans else ops.transpose(Xt)
+        if self.wide:   # S^T scatters
+            out = self._spread(XT, self.shape[1])
+        else:           # S^T (wide) selects
+            out = self._select(XT)
+        return _finish(ops.transpose(out), False, ref)
+
+    def toarray(self):
+        d = min(self.shape)
+        W = np.zeros((d, max(self.shape)))
+        W[np.arange(d), np.asarray(self.indices)] = 1.0
+        return W if self.wide else W.T.copy()
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.toarray()
+        return a.astype(dtype) if dtype is not None else a
+
+
+class SparseSketch:
+    """A general sparse operator (the sparse sign operator of
+    utils/sketching.py:83-103) held on the device as the CSR of S and of S^T;
+    S @ X and X @ S = (S^T X^T)^T run on the library's CSR product."""
+
+    __array_ufunc__ = None
+
+    def __init__(self, csr_s, csr_st, shape):
+        self._s, self._st = csr_s, csr_st
+        self.shape = tuple(shape)
+
+    @classmethod
+    def from_scipy(cls, S, device):
+        import scipy.sparse as sp
+
+        def up(M):
+            M = sp.csr_matrix(M)
+            M.sort_indices()
+            return (torch.from_numpy(M.indptr.astype(np.int64)).to(device),
+                    torch.from_numpy(M.indices.astype(np.int32)).to(device),
+                    torch.from_numpy(M.data.astype(np.float64)).to(device))
+        return cls(up(S), up(S.T), S.shape)
+
+    @property
+    def T(self):
+        return SparseSketch(self._st, self._s, (self.shape[1], self.shape[0]))
+
+    def toarray(self):
+        rp, cl, vl = (t.cpu().numpy() for t in self._s)
+        W = np.zeros(self.shape)
+        for i in range(self.shape[0]):
+            W[i, cl[rp[i]:rp[i + 1]]] += vl[rp[i]:rp[i + 1]]
+        return W
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.toarray()
+        return a.astype(dtype) if dtype is not None else a
+
+    def __matmul__(self, X):
+        if isinstance(X, (DenseSketch, SJLTSketch, SamplingSketch, SparseSketch)):
+            return NotImplemented
+        Xt, vec, ref = _prep(X)
+        if Xt.shape[0] != self.shape[1]:
+            raise ValueError(f"shape mismatch {self.shape} @ {tuple(Xt.shape)}")
+        rp, cl, vl = self._s
+        return _finish(ops.sjlt_left(rp, cl, vl, self.shape[0], Xt), vec, ref)
+
+    def __rmatmul__(self, X):
+        Xt, vec, ref = _prep(X)
+        if vec:
+            return _finish(self.T @ Xt, True, ref)
+        if Xt.shape[1] != self.shape[0]:
+            raise ValueError(f"shape mismatch {tuple(Xt.shape)} @ {self.shape}")
+        base, trans, _ = ops.mat_desc(Xt)
+        XT = base if trans else ops.transpose(Xt)
+        rp, cl, vl = self._st
+        return _finish(ops.transpose(ops.sjlt_left(rp, cl, vl, self.shape[1], XT)), False, ref)
 
 
 # ---------------------------------------------------------------------------
@@ -281,6 +373,78 @@ def orthonormal_operator(n_rows, n_cols, rng, device=None):
     return DenseSketch(ops.gemm(Q, D))
 
 
+def sparse_sign_operator(n_rows, n_cols, rng, density=0.05, device=None):
+    """utils/sketching.py:83-103: each entry nonzero with probability
+    `density` (redrawn up to ten times if all are zero), values
+    +-1/sqrt(min(r, c) density); the draws are the reference's numpy calls."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(rng)
+    nz = rng.random(n_rows * n_cols) < density
+    attempt = 0
+    while np.all(~nz):
+        if attempt == 10:
+            raise RuntimeError('Density too low.')
+        nz = rng.random(n_rows * n_cols) < density
+        attempt += 1
+    rows, cols = np.unravel_index(np.where(nz)[0], (n_rows, n_cols))
+    vals = np.ones(rows.size)
+    vals[rng.random(vals.size) < 0.5] = -1
+    vals /= np.sqrt(min(n_rows, n_cols) * density)
+    S = sp.coo_matrix((vals, (rows, cols)), shape=(n_rows, n_cols)).tocsr()
+    return SparseSketch.from_scipy(S, device or ops.default_device())
+
+
+def generate_srct(n_rows, n_cols, rng):
+    """utils/sketching.py:106-116: restriction r, signs e (scaled by
+    sqrt(big / small)) and permutation perm, drawn in the reference's order."""
+    big, small = max(n_rows, n_cols), min(n_rows, n_cols)
+    r = rng.choice(big, size=small, replace=False)
+    e = rng.random(big)
+    e[e > 0.5] = 1.0
+    e[e != 1] = -1.0
+    e *= np.sqrt(big / small)
+    perm = rng.permutation(big)
+    return r, e, perm
+
+
+def _srct_matrix(r, e, perm, dev):
+    N, d = e.size, r.size
+    pinv = np.argsort(perm)
+    rt = torch.from_numpy(np.ascontiguousarray(r, dtype=np.int64)).to(dev)
+    et = torch.from_numpy(np.ascontiguousarray(e, dtype=np.float64)).to(dev)
+    pt = torch.from_numpy(np.ascontiguousarray(pinv, dtype=np.int64)).to(dev)
+    S = torch.empty((d, N), dtype=ops.F64, device=dev)
+    ops.check(ops.lib().pamd_srct_fill_f64(d, N, ops._p(rt), ops._p(et), ops._p(pt), ops._p(S), max(N, 1),
+                                           ops.stream(dev)), "pamd_srct_fill_f64")
+    return S
+
+
+def apply_srct(r, e, mat, perm=None, forward=True):
+    """utils/sketching.py:119-176: the SRCT (r, e, perm) or its adjoint applied
+    to the columns of mat, as the device product with the operator's rows."""
+    Xt = ops.as_device(mat)
+    perm = np.arange(np.asarray(e).size) if perm is None else perm
+    S = DenseSketch(_srct_matrix(np.asarray(r), np.asarray(e), np.asarray(perm), Xt.device))
+    out = (S @ Xt) if forward else (S.T @ Xt)
+    return ops.like_input(out, mat)
+
+
+def srct_operator(n_rows, n_cols, rng, device=None):
+    """utils/sketching.py:179-201: the SRCT as a device operator.  Its d rows
+    of the orthonormal DCT-II (composed with the signs and the permutation)
+    are generated on the GPU, so S @ X, X @ S and S.T run on the FP64 MFMA
+    GEMM; `sketch_data` holds (r, e, perm) as in the reference."""
+    rng = np.random.default_rng(rng)
+    r, e, perm = generate_srct(n_rows, n_cols, rng)
+    if n_cols < n_rows:
+        # the reference draws before it looks at the orientation; the tall
+        # operator is the transpose of a second, wide draw (sketching.py:185-200)
+        return srct_operator(n_cols, n_rows, rng, device).T
+    S = DenseSketch(_srct_matrix(r, e, perm, device or ops.default_device()))
+    S.sketch_data = (r, e, perm)
+    return S
+
+
 def sampling_operator(n_rows, n_cols, rng, indices=None):
     """utils/sketching.py:204-232 (wide selection; the tall form is its transpose)."""
     rng = np.random.default_rng(rng)
@@ -300,7 +464,7 @@ def apply(X, Y):
     the left of a sketch operator, or two device tensors, go through the
     library's kernels rather than torch's.
     """
-    ops_types = (DenseSketch, SJLTSketch, SamplingSketch)
+    ops_types = (DenseSketch, SJLTSketch, SamplingSketch, SparseSketch)
     if getattr(X, "device_operator", False):  # split operator: sketches see its FP64 matrix
         return Y.__rmatmul__(X.dense()) if isinstance(Y, ops_types) else X @ Y
     if getattr(Y, "device_operator", False):
@@ -322,6 +486,6 @@ def as_tensor(S):
     """Dense device tensor for an operator or array (operators are densified)."""
     if isinstance(S, DenseSketch):
         return S.S
-    if isinstance(S, (SJLTSketch, SamplingSketch)):
+    if isinstance(S, (SJLTSketch, SamplingSketch, SparseSketch)):
         return ops.as_device(S.toarray())
     return ops.as_device(S)

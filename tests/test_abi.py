@@ -64,8 +64,8 @@ def test_argument_validation_without_gpu(built):
     rc = lib.pamd_gemm_f64(0, 0, 4, 4, 4, 1.0, ctypes.c_void_p(8), 2, ctypes.c_void_p(8), 4, 0.0,
                            ctypes.c_void_p(8), 4, None, 0, None)
     assert rc == -8
-    # trsv beyond the LDS-resident size
-    rc = lib.pamd_trsv_upper_f64(0, 100000, ctypes.c_void_p(8), 100000, ctypes.c_void_p(8), None)
+    # trsv: n must stay below 2^30 (beyond 2048 the vector is solved in place in global memory)
+    rc = lib.pamd_trsv_upper_f64(0, 1 << 30, ctypes.c_void_p(8), 1 << 30, ctypes.c_void_p(8), None)
     assert rc == -2
     b = ctypes.c_int64(0)
     assert lib.pamd_geqrf_workspace(10, 20, 20, ctypes.byref(b)) == -2  # n > m

@@ -503,3 +503,180 @@ def test_split_product_matches_exact_rational_sum():
         full = L @ R.T
         # dropped products and piece truncation are below 2^-(8P-1) |L| |R|^T elementwise
         assert np.all(np.abs(C - full) <= 2.0 ** -(8 * P - 2) * (np.abs(L) @ np.abs(R).T))
+
+
+# --------------------------------------------------------------------------
+# the drivers next to the hot path
+# --------------------------------------------------------------------------
+
+def _sign_fix(X, ref, axis=0):
+    """X with the sign of each column (axis 0) or row (axis 1) chosen to match ref."""
+    if axis == 0:
+        s = np.sign(np.sum(X * ref, axis=0))
+        return X * s
+    s = np.sign(np.sum(X * ref, axis=1))
+    return X * s[:, None]
+
+
+def test_spo_chol_and_svd_modes_match_reference_runs(golden):
+    A, b = golden["spo_A"], golden["spo_b"]
+    for tag, kind, mode in [("ga_chol", "gaussian", "chol"), ("sj_chol_ridge", "sjlt", "chol"),
+                            ("ga_svd", "gaussian", "svd"), ("sj_svd_ridge", "sjlt", "svd")]:
+        sf, delta = golden[f"spo_{tag}_meta"]
+        x, log = orc.spo(A, b, float(delta), 1e-12, 60, np.random.default_rng(5), sketch=kind,
+                         sampling_factor=float(sf), mode=mode)
+        np.testing.assert_allclose(x, golden[f"spo_{tag}_x"], rtol=1e-9, err_msg=tag)
+        ref_err = golden[f"spo_{tag}_errors"]
+        assert log.errors.size == ref_err.size, tag
+        # entries at the rounding floor (~1e-15 absolute) differ with the order of operations
+        np.testing.assert_allclose(log.errors, ref_err, rtol=1e-6, atol=1e-12 * ref_err[0], err_msg=tag)
+
+
+def test_spo_svd_rank_deficient_matches_reference(golden):
+    A, b = golden["spo_rd_A"], golden["spo_rd_b"]
+    x, log, info = orc.spo(A, b, 0.0, 1e-12, 60, np.random.default_rng(6), sketch="gaussian",
+                           sampling_factor=3, mode="svd", return_info=True)
+    assert info["R"].shape == (16, 12)  # the null directions are dropped
+    np.testing.assert_allclose(x, golden["spo_rd_svd_x"], rtol=1e-9)
+    ref = golden["spo_rd_svd_errors"]
+    np.testing.assert_allclose(log.errors, ref, rtol=1e-6, atol=1e-12 * ref[0])
+    # the minimum-norm LS solution (x lies in the row space)
+    np.testing.assert_allclose(x, la.lstsq(A, b)[0], rtol=1e-9)
+
+
+def test_svd_right_precond_matches_reference(golden):
+    A = golden["spo_A"]
+    M, U, sig, Vh = orc.svd_right_precond(orc.gaussian_operator(80, 300, np.random.default_rng(8)) @ A)
+    np.testing.assert_allclose(sig, golden["svdpc_sigma"], rtol=1e-12)
+    np.testing.assert_allclose(_sign_fix(M, golden["svdpc_M"]), golden["svdpc_M"], rtol=1e-10, atol=1e-10)
+    # the preconditioned sketch is orthonormal
+    SA = orc.gaussian_operator(80, 300, np.random.default_rng(8)) @ A
+    np.testing.assert_allclose((SA @ M).T @ (SA @ M), np.eye(M.shape[1]), atol=1e-11)
+
+
+def test_pcss2_underdetermined_matches_reference(golden):
+    A, R, c = golden["spo_A"], golden["pcss2u_R"], golden["pcss2u_c"]
+    x, y, errors, _ = orc.pcss2_under(A, c, 0.5, 1e-12, 60, R, True)
+    np.testing.assert_allclose(x, golden["pcss2u_x"], rtol=1e-9)
+    np.testing.assert_allclose(y, golden["pcss2u_y"], rtol=1e-9)
+    ref = golden["pcss2u_errors"]
+    np.testing.assert_allclose(errors, ref, rtol=1e-6, atol=1e-12 * ref[0])
+    # closed form: (A^T A + delta I) x = -c and y = -A x
+    n = A.shape[1]
+    x_opt = la.solve(A.T @ A + 0.5 * np.eye(n), -c)
+    np.testing.assert_allclose(x, x_opt, rtol=1e-9)
+    np.testing.assert_allclose(y, -A @ x_opt, rtol=1e-9, atol=1e-12)
+
+
+def test_spu1_minimum_norm_solution():
+    # test_underdet_least_squares.py: y_opt = U diag(1/s) V^T c solves min ||y|| s.t. A^T y = c
+    rng = np.random.default_rng(2)
+    for m, n, rank in ((1000, 50, 50), (800, 40, 25)):
+        U = orc.orthonormal_operator(m, rank, rng)
+        Vt = orc.orthonormal_operator(rank, n, rng)
+        s = np.geomspace(1, 1e-5, rank)
+        A = (U * s) @ Vt
+        c = A.T @ rng.standard_normal(m)
+        y_opt = U @ ((1 / s) * (Vt @ c))
+        y, log = orc.spu1(A, c, 1e-12, 100, np.random.default_rng(3))
+        assert la.norm(y - y_opt) / (1 + la.norm(y_opt)) <= 1e-8
+        assert la.norm(A.T @ y - c) <= 1e-8 * la.norm(c)
+        assert log.errors[-1] <= 1e-10 * log.errors[0]
+
+
+def test_sso1_matches_reference(golden):
+    A, b = golden["spo_A"], golden["spo_b"]
+    for tag, delta in (("", 0.0), ("_ridge", 0.3)):
+        x = orc.sso1(A, b, delta, np.random.default_rng(12))
+        np.testing.assert_allclose(x, golden[f"sso1{tag}_x"], rtol=1e-10)
+
+
+def test_pcss1_and_saddle_drivers_match_reference(golden):
+    A, b, c, M = golden["sad_A"], golden["sad_b"], golden["sad_c"], golden["sad_M"]
+    n = A.shape[1]
+    for tag, delta in (("", 0.0), ("_ridge", 0.2)):
+        x, y, res = orc.pcss1(A, b, c, delta, 1e-10, 50, M, False, None)
+        np.testing.assert_allclose(x, golden[f"pcss1{tag}_x"], rtol=1e-9)
+        np.testing.assert_allclose(y, golden[f"pcss1{tag}_y"], rtol=1e-9)
+        ref = golden[f"pcss1{tag}_res"]
+        np.testing.assert_allclose(res, ref, rtol=1e-6, atol=1e-12 * ref[0])
+        # the saddle-point solution: (A^T A + delta I) x = A^T b - c, y = b - A x.  M comes
+        # from the unregularised sketch, so with delta > 0 the 50 steps stop short (as in
+        # the reference); the residual then equals the last logged one
+        G, rhs = A.T @ A + delta * np.eye(n), A.T @ b - c
+        if delta == 0:
+            assert la.norm(G @ x - rhs) <= 1e-9 * la.norm(rhs)
+        np.testing.assert_allclose(y, b - A @ x, rtol=1e-12, atol=1e-14)
+        for drv in ("sps1", "sps2"):
+            fn = orc.sps1 if drv == "sps1" else orc.sps2
+            kw = dict(solver=orc.pcss1) if drv == "sps1" else {}
+            x, y, log = fn(A, b, c, delta, 1e-10, 50, np.random.default_rng(21), sampling_factor=4, **kw)
+            np.testing.assert_allclose(x, golden[f"{drv}{tag}_x"], rtol=1e-8, err_msg=drv + tag)
+            np.testing.assert_allclose(y, golden[f"{drv}{tag}_y"], rtol=1e-8, atol=1e-12, err_msg=drv + tag)
+            ref = golden[f"{drv}{tag}_errors"]
+            assert log.errors.size == ref.size, drv + tag
+            np.testing.assert_allclose(log.errors, ref, rtol=1e-5, atol=1e-12 * ref[0], err_msg=drv + tag)
+    x, y, log = orc.sps1(A, b, c, 0.1, 1e-10, 50, np.random.default_rng(22), solver=orc.pcss1,
+                         sampling_factor=0.6)
+    np.testing.assert_allclose(x, golden["sps1nys_x"], rtol=1e-8)
+    ref = golden["sps1nys_errors"]
+    np.testing.assert_allclose(log.errors, ref, rtol=1e-5, atol=1e-12 * ref[0])
+
+
+def test_qb3_and_evd2_match_reference(golden):
+    A = golden["qb_A"]
+    sk = lambda M, k, rng: orc.rs1(M, k, rng, None, 2, orc.orth, 1)
+    for tag, blk, tol in (("blk5", 5, 0.0), ("blk7tol", 7, 1e-3)):
+        Q, B = orc.qb3(A, 20, tol, np.random.default_rng(31), sk, blk)
+        Qr, Br = golden[f"qb3_{tag}_Q"], golden[f"qb3_{tag}_B"]
+        assert Q.shape == Qr.shape
+        np.testing.assert_allclose(_sign_fix(Q, Qr), Qr, atol=1e-9)
+        np.testing.assert_allclose(_sign_fix(B, Br, axis=1), Br, atol=1e-9)
+    U = golden["qb_U"]
+    Apsd = (U * np.geomspace(1, 1e-4, 30)) @ U.T
+    for tag, npass in (("p1", 1), ("p3", 3)):
+        sk = lambda M, k, rng, q=npass - 1: orc.rs1(M, k, rng, None, q, orc.orth, 1)
+        V, lamb = orc.evd2(Apsd, 12, np.nan, 6, np.random.default_rng(41), sk)
+        np.testing.assert_allclose(lamb, golden[f"evd2_{tag}_lamb"], rtol=1e-9)
+        Vr = golden[f"evd2_{tag}_V"]
+        np.testing.assert_allclose(_sign_fix(V, Vr), Vr, atol=1e-8)
+
+
+@pytest.mark.parametrize("maker", ["tall_low_exact_rank", "fast_decay_low_exact_rank", "wide_full_exact_rank"])
+def test_qb2_closed_forms(maker):
+    # test_qb.py: exact-rank recovery, B = Q^T A (on the original A), and the tolerance guarantee
+    A, U, s, Vt = getattr(mm, maker)()
+    rf = lambda M, k, tol, rng: orc.rf1(M, k, tol, rng, lambda MM, kk, g: orc.rs1(MM, kk, g, None, 2, orc.orth, 1))
+    for blk in (2, 5):
+        Q, B = orc.qb2(A, s.size, 0.0, np.random.default_rng(7), rf, blk)
+        assert la.norm(Q.T @ Q - np.eye(Q.shape[1])) <= 1e-10
+        assert la.norm(A - Q @ B) <= 1e-8
+        assert la.norm(B - Q.T @ A) <= 1e-8
+        tol = 1e-2
+        Q, B = orc.qb2(A, min(A.shape), tol, np.random.default_rng(8), rf, blk)
+        assert la.norm(A - Q @ B) <= tol * la.norm(A) * (1 + 1e-8)
+
+
+def test_srct_and_sparse_sign_match_reference(golden):
+    rng = np.random.default_rng(3)
+    r, e, perm = orc.generate_srct(12, 50, rng)
+    for a, k in ((r, "srct_r"), (e, "srct_e"), (perm, "srct_perm")):
+        np.testing.assert_array_equal(a, golden[k])
+    fwd = orc.apply_srct(r, e, golden["srct_X"], perm)
+    np.testing.assert_allclose(fwd, golden["srct_fwd"], rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(orc.apply_srct(r, e, fwd, perm, forward=False), golden["srct_adj"],
+                               rtol=1e-13, atol=1e-14)
+    St = orc.srct_matrix(40, 9, np.random.default_rng(5))
+    np.testing.assert_allclose(St @ golden["srct_tall_X"], golden["srct_tall_out"], rtol=1e-12, atol=1e-13)
+    # the DCT-II closed form: S[i, j] = s(r_i) cos(pi r_i (2 q + 1) / (2N)) e_q with q = pinv[j]
+    N = 50
+    S = orc.apply_srct(r, e, np.eye(N), perm)
+    q = np.argsort(perm)
+    scale = np.where(r == 0, np.sqrt(1.0 / N), np.sqrt(2.0 / N))
+    closed = scale[:, None] * np.cos(np.pi * r[:, None] * (2 * q[None, :] + 1) / (2 * N)) * e[q][None, :]
+    np.testing.assert_allclose(S, closed, atol=1e-14)
+    # isometry on average: S^T S / N has unit diagonal in expectation; rows are orthonormal times big/small
+    np.testing.assert_allclose(S @ S.T, (N / 12) * np.eye(12), atol=1e-12)
+    for tag, (rr, cc, seed, dens) in {"ss20x60": (20, 60, 7, 0.1), "ss50x8": (50, 8, 8, 0.3)}.items():
+        np.testing.assert_array_equal(orc.sparse_sign_operator(rr, cc, np.random.default_rng(seed), dens),
+                                      golden[f"sps_{tag}"])
