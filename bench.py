@@ -28,10 +28,40 @@ sys.path.insert(0, ROOT)
 METRIC = "LS solves/s at ≤1e-10 backward err (chosen vs plain FP64 GPU); QB GB/s"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2516.6
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def roofline_terms(terms, measured_s):
+    """Roofline time of a solve from its algorithmic work: terms is a list of
+    (name, amount, rate) with amount in flop or bytes and rate in flop/s or
+    B/s at the published peak.  Returns the dict the bench reports."""
+    parts = {name: amount / rate * 1e3 for name, amount, rate in terms}
+    t_roof = sum(parts.values())
+    return {"roofline_ms": round(t_roof, 4), "measured_ms": round(measured_s * 1e3, 4),
+            "frac": round(t_roof / 1e3 / measured_s, 4),
+            "parts_ms": {k: round(v, 4) for k, v in parts.items()}}
+
+
+def device_eta(ops, torch, A, b, x):
+    """eta = ||A^T r|| / (||A||_F ||r||) from the library's compensated
+    residual pass (r and A^T r to about working precision)."""
+    g, rss, _ = ops.Residual(A)(x, b)
+    return float((torch.linalg.norm(g) / (torch.linalg.norm(A) * torch.sqrt(rss))).item())
 
 
 def cpu_threads():
@@ -84,6 +114,11 @@ def main():
                     help="only run the two BF16 MFMA GEMMs once each: the split GEMM on the QB1 shape "
                          "A S (8192 x 266, K 4096, 2 pieces) and the exact sketch at A 262144 x 1024 "
                          "(d 4096, 4 pieces) (the workload of the rocprofv3 SQ counter passes)")
+    ap.add_argument("--profile-ls", action="store_true",
+                    help="only run 10 plain FP64 SPO solves on A 20000 x 200 (the workload of the LS kernel trace)")
+    ap.add_argument("--profile-batched", action="store_true",
+                    help="only run the batched solve of the 512 uneven problems twice (the workload of the batched "
+                         "kernel trace)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
     if args.gpus != 1:
@@ -122,6 +157,25 @@ def main():
         Q = ops.gauss_q(4096, 262144, 1, dev)
         ops.ozaki_sketch(Q, 4096, ops.OzakiSplit(A, b, 4), 1.0)
         torch.cuda.synchronize()
+        return
+
+    if args.profile_ls:
+        Al_h, bl_h, _ = mm.config_ls_plain()
+        Al, bl = torch.from_numpy(Al_h).to(dev), torch.from_numpy(bl_h).to(dev)
+        spo = pa.SPO(pa.SkOpGA(backend="device"), 4)
+        for i in range(12):
+            spo(Al, bl, 0.0, 1e-12, 100, i, logging=False)
+        torch.cuda.synchronize()
+        log("profile-ls: 12 plain FP64 SPO solves done")
+        return
+    if args.profile_batched:
+        from parla_amd.drivers import batched_ls as bl_
+        As4, bs4 = bl_.uneven_batch(args.batch_problems, mm.data_seed(4), dev)
+        solver = bl_.BatchedSPO(cg_iters=20)
+        for i in range(2):
+            solver(As4, bs4, 1e-10, 1 + i)
+        torch.cuda.synchronize()
+        log("profile-batched: 2 batched solves done")
         return
 
     # ------------------------------------------------------------------ QB
@@ -210,6 +264,17 @@ def main():
     eta = orc.backward_error_extended(Al_h, bl_h, x_ls)
     _, lg = spo(Al, bl, 0.0, 1e-12, 100, 7, logging=True)
     ls_iters = int(lg.errors.size - 1)
+    ml, nl, dl = Al_h.shape[0], Al_h.shape[1], 4 * Al_h.shape[1]
+    # the algorithmic work of one solve as the survey of the reference counts it: S (device-drawn,
+    # written and read once), S A on FP64 MFMA, Householder QR of [SA | Sb], and per LSQR
+    # iteration the two products A v and A^T u (66 MB; the fused kernel reads A once instead)
+    ls_roof = roofline_terms([
+        ("gaussian_S_write_read", 2 * 8.0 * dl * ml, HBM_PEAK_GBS * 1e9),
+        ("sketch_SA_f64", 2.0 * dl * ml * nl, FP64_PEAK_TFLOPS * 1e12),
+        ("qr_of_sketch", 2.0 * dl * (nl + 1) ** 2 - 2.0 * (nl + 1) ** 3 / 3, FP64_PEAK_TFLOPS * 1e12),
+        ("lsqr_two_passes_per_iter", ls_iters * 2 * 8.0 * ml * nl, HBM_PEAK_GBS * 1e9)], t_ls)
+    ls_roof["dominant_kernels"] = "lsqr_tail_kernel (one workgroup, latency) and lsqr_pass_kernel (HBM); " \
+                                  "see profiles/ls_plain_20000x200_kernel_stats.md"
 
     # the chosen path on the same problem: exact BF16 sketch + FP64 refinement
     # until eta <= 1e-10 (the metric's threshold)
@@ -246,9 +311,47 @@ def main():
         log("C3: plain FP64 SPO on the same problem")
         t_p3 = timed_steps(plain3_step, 2, 1, torch)
         x_p3 = out["P3"][0].cpu().numpy()
+        _, lgp3 = plain3(A3, b3, 0.0, 1e-12, 100, 7, logging=True)
         log("C3: backward errors in extended precision on the host")
         eta_mx = orc.backward_error_extended(A3h, b3h, x_mx)
         eta_p3 = orc.backward_error_extended(A3h, b3h, x_p3)
+        m3, n3, d3 = A3h.shape[0], A3h.shape[1], 4096
+        steps3 = int(len(lgm.errors) - 1)
+        qr3 = 2.0 * d3 * (n3 + 1) ** 2 - 2.0 * (n3 + 1) ** 3 / 3
+        mx_roof = roofline_terms([
+            ("exact_sketch_4_bf16_pieces", 4 * 2.0 * d3 * m3 * (n3 + 1), BF16_PEAK_TFLOPS * 1e12),
+            ("qr_of_sketch", qr3, FP64_PEAK_TFLOPS * 1e12),
+            ("B_eq_A_Rinv_f64", float(m3) * n3 * n3, FP64_PEAK_TFLOPS * 1e12),
+            ("B_written_f32", 4.0 * m3 * n3, HBM_PEAK_GBS * 1e9),
+            ("residual_passes_f64", (steps3 + 1) * 8.0 * m3 * n3, HBM_PEAK_GBS * 1e9),
+            ("cg_normal_passes_f32", int(lgm.inner_iterations) * 4.0 * m3 * n3, HBM_PEAK_GBS * 1e9)], t_mx)
+        mx_roof["dominant_kernels"] = "oz_gemm_kernel (BF16 MFMA), trsm leaves + gemm_f64 (B = A R^-1), " \
+                                      "normal_pass_kernel (HBM); see profiles/ls_mixed_262144x1024_kernel_stats.md"
+        iters_p3 = int(lgp3.errors.size - 1)
+        p3_roof = roofline_terms([
+            ("gaussian_S_write_read", 2 * 8.0 * d3 * m3, HBM_PEAK_GBS * 1e9),
+            ("sketch_SA_f64", 2.0 * d3 * m3 * n3, FP64_PEAK_TFLOPS * 1e12),
+            ("qr_of_sketch", qr3, FP64_PEAK_TFLOPS * 1e12),
+            ("lsqr_two_passes_per_iter", iters_p3 * 2 * 8.0 * m3 * n3, HBM_PEAK_GBS * 1e9)], t_p3)
+        cpu3 = None
+        if not args.no_cpu:
+            # bounded CPU sample: the same family at 1/8 of the rows (the sketch, the LSQR passes and
+            # the draws scale with m; the 4096 x 1024 QR does not, so the extrapolation is generous)
+            m3s = m3 // 8
+            log(f"C3: CPU oracle SPO on a {m3s} x {n3} sample of the family")
+            A3s, b3s, _ = mm.config_ls_mixed(m=m3s)
+            ts3 = []
+            for i in range(3):
+                t0 = time.perf_counter()
+                x3s, _ = orc.spo(A3s, b3s, 0.0, 1e-12, 100, np.random.default_rng(i), sketch="gaussian",
+                                 sampling_factor=4, logging=False)
+                ts3.append(time.perf_counter() - t0)
+            t3s = float(np.median(ts3))
+            cpu3 = {"sample": f"A {m3s}x{n3} of the same family (cond 1e8), Gaussian d=4096, LSQR tol 1e-12; "
+                              "median of 3", "sample_s": round(t3s, 3),
+                    "full_size_solves_per_s_est": round(1.0 / (8 * t3s), 5), "cores": cpu_threads(),
+                    "cpu": cpu_model(), "sample_eta": float(orc.backward_error(A3s, b3s, x3s))}
+            del A3s, b3s
         mixed_extra = {
             "ls_mixed_workload": "A 262144x1024 f64, s geometric 1..1e-8, b = A x* + 1e-8 N(0,1); exact BF16 "
                                  "Gaussian sketch d=4096 (4 BF16 pieces of A), FP64 QR, B = A R^-1 stored FP32, "
@@ -260,8 +363,10 @@ def main():
                                   "presolve": round(lgm.time_presolve * 1e3, 3),
                                   "refine": round(lgm.time_iterate * 1e3, 3)},
             "ls_plain_f64_c3_solves_per_s": round(1.0 / t_p3, 3), "ls_plain_f64_c3_ms": round(t_p3 * 1e3, 3),
-            "ls_plain_f64_c3_eta": eta_p3,
+            "ls_plain_f64_c3_eta": eta_p3, "ls_plain_f64_c3_lsqr_iters": iters_p3,
             "ls_mixed_speedup_vs_plain_f64": round(t_p3 / t_mx, 3),
+            "ls_mixed_roofline": mx_roof, "ls_plain_f64_c3_roofline": p3_roof,
+            "ls_c3_cpu_baseline": cpu3,
         }
         del A3, b3
 
@@ -346,20 +451,43 @@ def main():
         xs4, lg4 = solver(As4, bs4, 1e-10, 2)
         torch.cuda.synchronize()
         t_b4 = time.perf_counter() - t0
-        log("C4: plain FP64 SPO one problem at a time on a sample of 16 problems")
-        idx = np.random.default_rng(3).choice(P4, min(16, P4), replace=False)
+        # the plain FP64 path, one problem at a time, stopped at the metric's threshold: the loosest
+        # LSQR tolerance for which every sampled solve reaches eta <= 1e-10 (checked by the
+        # compensated residual), then timed at that tolerance
+        n_s4 = min(64, P4)
+        idx = np.random.default_rng(3).choice(P4, n_s4, replace=False)
+        A4d = {int(i): As4[i].double() for i in idx}
         plain4 = pa.SPO(pa.SkOpSJ(8, backend="device"), 4)
+        tol4, etas_p4 = None, None
+        for cand in (1e-10, 3e-11, 1e-11, 3e-12, 1e-12):
+            log(f"C4: plain FP64 SPO on {n_s4} sampled problems at LSQR tol {cand:g}")
+            e_c = []
+            for i in idx:
+                xi, _ = plain4(A4d[int(i)], bs4[i], 0.0, cand, 400, int(i), logging=False)
+                e_c.append(device_eta(ops, torch, A4d[int(i)], bs4[i], xi))
+            tol4, etas_p4 = cand, e_c
+            if max(e_c) <= 1e-10:
+                break
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in idx:
-            plain4(As4[i].double(), bs4[i], 0.0, 1e-12, 200, int(i), logging=False)
+            plain4(A4d[int(i)], bs4[i], 0.0, tol4, 400, int(i), logging=False)
         torch.cuda.synchronize()
         t_p4 = (time.perf_counter() - t0) / len(idx)
+        del A4d
         etas = []
         for i in idx[:6]:
             etas.append(orc.backward_error_extended(As4[i].double().cpu().numpy(), bs4[i].cpu().numpy(),
                                                     xs4[i].cpu().numpy()))
-        work = sum(A.shape[0] * A.shape[1] for A in As4)
+        mn4 = np.array([A.shape[0] * A.shape[1] for A in As4], dtype=np.float64)
+        work = int(mn4.sum())
+        # algorithmic HBM bytes of the batched solve: the FP32 A_i read by the sketch, by the
+        # residual at x = 0, and per refinement step by 20 normal passes and one residual pass
+        # of every problem still active
+        et4 = lg4["etas"]
+        pass_elems = 2.0 * mn4.sum() + sum(float(mn4[et4[s] > 1e-10].sum()) * 21 for s in range(len(et4) - 1))
+        b4_roof = roofline_terms([("fp32_passes_over_A_i", 4.0 * pass_elems, HBM_PEAK_GBS * 1e9)], t_b4)
+        b4_roof["dominant_kernels"] = "bt_normal_kernel (HBM); see profiles/ls_batched_512_kernel_stats.md"
         batched_extra = {
             "batched_workload": f"{P4} problems, m_i U[2000,50000], n_i U[32,512], m_i >= 8 n_i, cond log-U[1e2,1e10], "
                                 "FP32 A_i; SJLT d_i = 4 n_i, shifted CholeskyQR3, graph-replayed FP64 refinement "
@@ -371,7 +499,12 @@ def main():
             "batched_eta_extended_sample": [float(e) for e in etas],
             "batched_sum_mn": int(work),
             "plain_f64_one_at_a_time_solves_per_s": round(1.0 / t_p4, 2),
+            "plain_f64_sample": f"{n_s4} problems, SJLT SPO at LSQR tol {tol4:g} (the loosest of 1e-10 ... 1e-12 "
+                                "at which every sampled eta <= 1e-10)",
+            "plain_f64_lsqr_tol": tol4, "plain_f64_max_eta": float(max(etas_p4)),
+            "plain_f64_etas": [float(e) for e in etas_p4],
             "batched_speedup_vs_plain_f64": round((P4 / t_b4) * t_p4, 2),
+            "batched_roofline": b4_roof,
             "batched_tuned_shapes": {f"{k[0]}/npl{k[1]}": list(v) for k, v in lg4["tuned"].items()},
         }
         del As4, bs4, xs4
@@ -393,9 +526,13 @@ def main():
             ts.append(time.perf_counter() - t0)
         t_cpu = float(np.median(ts[1:]))
         cpu = {"value": qb_bytes / t_cpu / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+               "cpu": cpu_model(),
                "sample": "full QB1 workload (8192 x 4096, k = 266, SJLT drawn with numpy), "
                          "1 warm-up + median of 3"}
         extra_cpu["qb_oracle_rel_err"] = float(np.linalg.norm(A_host - Qo @ Bo) / fro)
+        # SVD1's singular values from the oracle's QB: the same algorithmic error as the device's
+        so = np.linalg.svd(Bo, compute_uv=False)[:k]
+        extra_cpu["svd1_oracle_max_rel_err_top256"] = float(np.max(np.abs(so - s_true[:k]) / s_true[:k]))
         log("CPU oracle SPO on A 20000 x 200")
         ts = []
         for i in range(4):
@@ -438,7 +575,7 @@ def main():
                                  "FP64 inside the timed step",
             "svd1_ms": round(t_svd * 1e3, 3), "svd1_max_rel_err_top256": sv_rel,
             "ls_plain_f64_solves_per_s": round(1.0 / t_ls, 3), "ls_plain_f64_ms": round(t_ls * 1e3, 3),
-            "ls_plain_f64_eta": eta, "ls_plain_f64_iters": ls_iters,
+            "ls_plain_f64_eta": eta, "ls_plain_f64_iters": ls_iters, "ls_plain_f64_roofline": ls_roof,
             "ls_chosen_solves_per_s": round(1.0 / t_lsc, 3), "ls_chosen_ms": round(t_lsc * 1e3, 3),
             "ls_chosen_eta": eta_c,
             "ls_chosen_workload": "same problem; SPOMixed: exact BF16 Gaussian sketch d=800 (4 pieces), FP64 QR, "

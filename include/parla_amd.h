@@ -201,6 +201,40 @@ int pamd_scale_cols_f64(int64_t rows, int64_t cols, const double* A, int64_t lda
                         const double* s, int invert, double* out, int64_t ldo,
                         void* stream);
 
+/* Workspace (bytes) for pamd_svd_jacobi_f64 (1 <= n <= m <= 1024). */
+int pamd_svd_jacobi_workspace(int64_t m, int64_t n, int64_t* bytes);
+
+/* Thin SVD A = U diag(s) Vt of A (m x n, n <= m <= 1024) by one-sided block
+ * Jacobi: U (m x n, may be NULL), s (n, descending), Vt (n x n, rows are the
+ * right singular vectors, may be NULL).  Columns of U for zero singular values
+ * are zero.  Up to max_sweeps (<= 64) sweeps are enqueued; sweeps after the
+ * first that rotates nothing return at once, so nothing is read back.
+ * info[0] (device int) = 0, or 1 when the last sweep still rotated.
+ * Replaces la.svd at drivers/svd.py:164, comps/preconditioning.py:74 and the
+ * SVD-based la.lstsq of drivers/least_squares.py:186 (on the triangular
+ * factor of a Householder QR when the matrix is larger). */
+int pamd_svd_jacobi_f64(int64_t m, int64_t n, const double* A, int64_t lda,
+                        double* U, int64_t ldu, double* s, double* Vt, int64_t ldvt,
+                        int max_sweeps, void* workspace, int* info, void* stream);
+
+/* out (n x n) = C + shift * I for the symmetric C whose lower triangle A
+ * holds (the upper triangle of A is not read, as scipy.linalg.eigh).  First
+ * step of the eigendecomposition that replaces la.eigh at drivers/evd.py:279. */
+int pamd_sym_shift_f64(int64_t n, const double* A, int64_t lda, double shift,
+                       double* out, int64_t ldo, void* stream);
+
+/* Eigenvalues of a symmetric C from the SVD C + shift * I = U diag(s) Vt:
+ * lamb_j = s_j sign(u_j . v_j) - shift (u_j column j of U, v_j row j of Vt;
+ * the eigenvectors are the v_j).  info[0] (device int, may be NULL) is raised
+ * to 2 when, for some s_j > 1e-10 s_0, |u_j . v_j| differs from 1 by more than
+ * 1e-6: an eigenvalue pair lambda, -lambda of nearly equal magnitude mixed the
+ * singular vectors, and the SVD must be redone with shift >= max |lambda|.
+ * lamb must not alias s.  Second step of the la.eigh replacement
+ * (drivers/evd.py:279). */
+int pamd_eig_from_svd_f64(int64_t n, const double* U, int64_t ldu, const double* Vt,
+                          int64_t ldvt, const double* s, double shift, double* lamb,
+                          int* info, void* stream);
+
 /* out (cols x rows) = A^T for A (rows x cols); plumbing for A.T @ S products. */
 int pamd_transpose_f64(int64_t rows, int64_t cols, const double* A, int64_t lda,
                        double* B, int64_t ldb, void* stream);

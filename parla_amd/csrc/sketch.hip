@@ -7,9 +7,10 @@
 // atomics:
 //   left   out[i, :] = sum_{(j, v) in row i of W} v * A[j, :]     (W @ A)
 //   right  out[r, c] = sum_{(j, v) in row c of W} v * A[r, j]     (A @ W^T)
-// The right product stages a 16-row x 512-column tile of A in LDS and walks
-// each output column's CSR list through the tiles, so W's index arrays are
-// read once per 16 rows of A rather than once per row.
+// The right product (k <= 512) stages 32-row tiles of A in LDS, transposed,
+// with one lane per row and each output column's CSR list walked by a
+// half-wavefront, so W's index arrays are read once per 32 rows of A; wider
+// outputs use one thread per output column over 16-row tiles.
 #include "common.h"
 #include "sjlt_dev.h"
 
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(1024) void sjlt_right_kernel(int64_t m, int64_t n, 
         if (jv[e] < j1) {  // a prefix of the eight (sorted entries)
           const int jl = jv[e] - (int)j0;
 #pragma unroll
-          for (int rr = 0; rr < RT_ROWS; ++rr) acc[rr] += vv[e] * As[rr][jl];
+          for (int rr = 0; rr < RT_ROWS; ++rr) acc[rr] = fma(vv[e], As[rr][jl], acc[rr]);
           used = e + 1;
         }
       }
@@ -203,6 +204,103 @@ __global__ __launch_bounds__(1024) void sjlt_right_kernel(int64_t m, int64_t n, 
 #pragma unroll
     for (int rr = 0; rr < RT_ROWS; ++rr)
       if (rr < nr) out[(r0 + rr) * ldo + c] = acc[rr];
+}
+
+// A * W^T for k <= 512 output columns, one lane per row of A.  A workgroup of
+// 1024 threads owns RS_ROWS = 32 rows of A and every output column: each
+// half-wavefront (32 lanes, lane = row) owns the output columns c = h, h + 32,
+// h + 64, ... and keeps their accumulators and CSR cursors in registers.  A
+// streams through LDS in tiles of 32 rows x RS_TJ columns, stored transposed
+// (column j of the tile is 32 consecutive doubles, one per row), so for an
+// entry (j, v) of column c the half-wave reads the 32 rows of column j in one
+// conflict-free 256-byte access and each lane does one FMA.  The next tile is
+// loaded into registers while the current one is used.  Every output sums its
+// entries in ascending j with one accumulator, the order of sjlt_right_kernel,
+// so both give the same bits.  A is read from HBM once; the CSR (index and
+// value arrays, L2-resident) once per workgroup.
+constexpr int RS_ROWS = 32;
+constexpr int RS_TJ = 256;
+constexpr int RS_THREADS = 1024;
+constexpr int RS_HALVES = RS_THREADS / 32;                   // 32 half-wavefronts
+constexpr int RS_PRE = RS_ROWS * RS_TJ / RS_THREADS;          // tile elements per thread
+
+template <int CPH>  // output columns per half-wavefront: k <= 32 * CPH
+__global__ __launch_bounds__(RS_THREADS) void sjlt_right_rows_kernel(int64_t m, int64_t n, int64_t k,
+                                                                     const int64_t* __restrict__ rowptr,
+                                                                     const int32_t* __restrict__ cols,
+                                                                     const double* __restrict__ vals,
+                                                                     const double* __restrict__ A, int64_t lda,
+                                                                     double* __restrict__ out, int64_t ldo) {
+  __shared__ double As[RS_TJ][RS_ROWS + 1];
+  const int tid = threadIdx.x;
+  const int r = tid & 31;           // row within the block (lane within the half-wave)
+  const int h = tid >> 5;           // half-wavefront
+  const int64_t r0 = (int64_t)blockIdx.x * RS_ROWS;
+  int p[CPH], pe[CPH];
+  double acc[CPH];
+#pragma unroll
+  for (int q = 0; q < CPH; ++q) {
+    const int64_t c = h + (int64_t)RS_HALVES * q;
+    p[q] = (c < k) ? (int)rowptr[c] : 0;
+    pe[q] = (c < k) ? (int)rowptr[c + 1] : 0;
+    acc[q] = 0.0;
+  }
+  // tile element e of this thread: row e / RS_TJ, column e % RS_TJ (coalesced along j)
+  double pre[RS_PRE];
+  auto load_tile = [&](int64_t j0) {
+#pragma unroll
+    for (int t = 0; t < RS_PRE; ++t) {
+      const int e = tid + RS_THREADS * t;
+      const int64_t row = r0 + e / RS_TJ, col = j0 + e % RS_TJ;
+      pre[t] = (row < m && col < n) ? A[row * lda + col] : 0.0;
+    }
+  };
+  load_tile(0);
+  for (int64_t j0 = 0; j0 < n; j0 += RS_TJ) {
+    __syncthreads();  // every half-wave is done with the previous tile
+#pragma unroll
+    for (int t = 0; t < RS_PRE; ++t) {
+      const int e = tid + RS_THREADS * t;
+      As[e % RS_TJ][e / RS_TJ] = pre[t];
+    }
+    __syncthreads();
+    if (j0 + RS_TJ < n) load_tile(j0 + RS_TJ);  // in flight while this tile is used
+    const int j1 = (int)imin64(n, j0 + RS_TJ);
+#pragma unroll
+    for (int q = 0; q < CPH; ++q) {
+      // the column's entries in [j0, j1) are a prefix of what remains (sorted
+      // by j); fetched four at a time, so the index loads are waited on once
+      // per four entries
+      while (p[q] < pe[q]) {
+        int jv[4];
+        double vv[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const bool in = p[q] + e < pe[q];
+          jv[e] = in ? cols[p[q] + e] : INT32_MAX;
+          vv[e] = in ? vals[p[q] + e] : 0.0;
+        }
+        int used = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (jv[e] < j1) {
+            acc[q] = fma(vv[e], As[jv[e] - (int)j0][r], acc[q]);
+            used = e + 1;
+          }
+        }
+        p[q] += used;
+        if (used < 4) break;  // the rest of this column lies in later tiles
+      }
+    }
+  }
+  const int64_t row = r0 + r;
+  if (row < m) {
+#pragma unroll
+    for (int q = 0; q < CPH; ++q) {
+      const int64_t c = h + (int64_t)RS_HALVES * q;
+      if (c < k) out[row * ldo + c] = acc[q];
+    }
+  }
 }
 
 __global__ void transpose_kernel(int64_t rows, int64_t cols, const double* __restrict__ A, int64_t lda,
@@ -289,6 +387,28 @@ extern "C" int pamd_sjlt_csr(int64_t n_short, int64_t n_long, int k, const int32
   return PAMD_OK;
 }
 
+namespace {
+
+// launch sjlt_right_rows_kernel<cph> (cph = ceil(k / 32) <= 16); false if not built for cph
+bool sjlt_right_rows_launch(int cph, int64_t m, int64_t n, int64_t k, const int64_t* rowptr, const int32_t* cols,
+                            const double* vals, const double* A, int64_t lda, double* out, int64_t ldo,
+                            hipStream_t st) {
+  const unsigned nb = (unsigned)pamd_cdiv(m, RS_ROWS);
+#define PAMD_RS(C)                                                                                         \
+  case C:                                                                                                  \
+    sjlt_right_rows_kernel<C><<<nb, RS_THREADS, 0, st>>>(m, n, k, rowptr, cols, vals, A, lda, out, ldo); \
+    return true;
+  switch (cph) {
+    PAMD_RS(1) PAMD_RS(2) PAMD_RS(3) PAMD_RS(4) PAMD_RS(5) PAMD_RS(6) PAMD_RS(7) PAMD_RS(8)
+    PAMD_RS(9) PAMD_RS(10) PAMD_RS(11) PAMD_RS(12) PAMD_RS(13) PAMD_RS(14) PAMD_RS(15) PAMD_RS(16)
+    default:
+      return false;
+  }
+#undef PAMD_RS
+}
+
+}  // namespace
+
 extern "C" int pamd_sjlt_left_f64(int64_t d, int64_t m, int64_t n, const int64_t* rowptr, const int32_t* cols,
                                   const double* vals, const double* A, int64_t lda, double* out, int64_t ldo,
                                   void* stream) {
@@ -320,6 +440,12 @@ extern "C" int pamd_sjlt_right_f64(int64_t m, int64_t n, int64_t k, const int64_
   PAMD_ARG_CHECK(ldo >= k && ldo >= 1, 10);
   if (m == 0 || k == 0) return PAMD_OK;
   hipStream_t st = (hipStream_t)stream;
+  const int cph = (int)pamd_cdiv(k, RS_HALVES);
+  if (cph <= 16 && k * n < (1LL << 31) && sjlt_right_rows_launch(cph, m, n, k, rowptr, cols, vals, A, lda, out, ldo,
+                                                                st)) {
+    PAMD_LAUNCH_CHECK();
+    return PAMD_OK;
+  }
   const unsigned nb = (unsigned)pamd_cdiv(m, RT_ROWS);
   const unsigned threads = (unsigned)(pamd_cdiv(k, 64) * 64);
   sjlt_right_kernel<<<nb, threads, 0, st>>>(m, n, k, rowptr, cols, vals, A, lda, out, ldo);

@@ -39,7 +39,10 @@ class EVD1(EVDecomposer):
         At = ops.as_device(A)
         Q, B = self.qb(At, k + over, tol / 2, rng)
         C = ops.gemm(B, Q)
-        lamb, U = torch.linalg.eigh(C, UPLO='L')
+        if C.shape[0] <= ops.JACOBI_MAX:
+            lamb, U = ops.sym_eig(C)    # the library's Jacobi, lower triangle (as eigh)
+        else:
+            lamb, U = torch.linalg.eigh(C, UPLO='L')
         alamb = np.abs(lamb.detach().cpu().numpy())
         d = Q.shape[1]
         r = min(k, d, int(np.count_nonzero(alamb > 10 * EPS)))

@@ -446,21 +446,87 @@ def svd_small(M):
     return Vb, s, Ub
 
 
+JACOBI_MAX = 1024          # largest dimension the LDS-resident Jacobi pair handles
+JACOBI_SWEEPS = 40
+
+
+def svd_jacobi(X):
+    """Thin SVD X = U diag(s) Vt of a device matrix X (m x n, n <= m <=
+    1024) by the library's one-sided block Jacobi; s descending.  Raises
+    numpy.linalg.LinAlgError when the sweeps do not converge (as la.svd)."""
+    m, n = X.shape
+    Xc = X if X.stride(-1) == 1 else X.contiguous()
+    dev = X.device
+    U = torch.empty((m, n), dtype=F64, device=dev)
+    s = torch.empty(n, dtype=F64, device=dev)
+    Vt = torch.empty((n, n), dtype=F64, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    ws = workspace(_query(lib().pamd_svd_jacobi_workspace, m, n), dev)
+    with _Timed("svd_jacobi", 0.0, 8.0 * (m * n + n * n)):
+        check(lib().pamd_svd_jacobi_f64(m, n, _p(Xc), max(Xc.stride(0), n, 1), _p(U), n, _p(s), _p(Vt), n,
+                                        JACOBI_SWEEPS, _p(ws), _p(info), stream(dev)), "pamd_svd_jacobi_f64")
+    if int(info.item()) != 0:
+        raise np.linalg.LinAlgError(f"Jacobi SVD did not converge in {JACOBI_SWEEPS} sweeps")
+    return U, s, Vt
+
+
 def thin_svd(X):
     """X = U diag(s) Vh (thin: k = min(d, n) singular triplets) of a device
-    matrix X (d x n).  Householder QR of the taller orientation, then the k x k
-    SVD of its triangular factor (svd_small): for d >= n, X = Q R and
-    R = Ur diag(s) Vh give U = Q Ur; for d < n, X^T = Q R gives X = R^T Q^T,
-    R^T = Ur diag(s) Vrh and Vh = (Q Vrh^T)^T.  The large factor stays on the
-    device.  Returns (U, s, Vh)."""
+    matrix X (d x n), s descending.
+
+    Preconditioned one-sided Jacobi (Drmac and Veselic): T, the taller
+    orientation of X (m x k), is factored T = Q R by Householder QR, and the
+    Jacobi SVD (svd_jacobi) runs on R^T, whose columns are far closer to
+    orthogonal than T's.  R^T = Uj diag(s) Vtj gives T = (Q Vtj^T) diag(s) Uj^T.
+    Beyond k = 1024 the k x k SVD of R is rocSOLVER's dgesdd (svd_small).
+    Returns (U, s, Vh)."""
     d, n = X.shape
+    T = X.contiguous().clone() if d >= n else transpose(X)
+    f = geqrf_(T)
+    R = torch.triu(f.R_view)
+    k = R.shape[0]
+    if k <= JACOBI_MAX:
+        Uj, s, Vtj = svd_jacobi(transpose(R))
+        UT, VhT = gemm(f.Q(), Vtj.t()), transpose(Uj)
+    else:
+        Ur, s, VhT = svd_small(R)
+        UT, VhT = gemm(f.Q(), Ur), VhT.contiguous()
     if d >= n:
-        f = geqrf_(X.contiguous().clone())
-        Ur, s, Vh = svd_small(torch.triu(f.R_view))
-        return gemm(f.Q(), Ur), s, Vh.contiguous()
-    f = geqrf_(transpose(X))
-    Ur, s, Vrh = svd_small(torch.triu(f.R_view).t())
-    return Ur.contiguous(), s, transpose(gemm(f.Q(), Vrh.t()))
+        return UT, s, VhT
+    return transpose(VhT), s, transpose(UT)    # X = T^T = VhT^T diag(s) UT^T
+
+
+def sym_eig(C):
+    """Eigendecomposition of the symmetric device matrix C (lower triangle
+    read, as scipy.linalg.eigh): (lamb, V), ordered by decreasing |lamb|.
+    The eigenvalues of C are its singular values with the signs of
+    u_j . v_j, so this is thin_svd (the preconditioned Jacobi SVD) plus one
+    small kernel.  When an eigenvalue pair of nearly equal magnitude and
+    opposite sign mixes the singular vectors, the decomposition is redone on
+    C + ||C||_2 I, which is positive semidefinite (the eigenvalues then carry
+    the absolute error u ||C||, as eigh's do)."""
+    n = C.shape[0]
+    Cc = C if C.stride(-1) == 1 else C.contiguous()
+    dev = C.device
+    lamb = torch.empty(n, dtype=F64, device=dev)
+    info = torch.zeros(1, dtype=torch.int32, device=dev)
+    Cs = torch.empty((n, n), dtype=F64, device=dev)
+
+    def run(shift):
+        check(lib().pamd_sym_shift_f64(n, _p(Cc), max(Cc.stride(0), n, 1), float(shift), _p(Cs), n, stream(dev)),
+              "pamd_sym_shift_f64")
+        U, s, Vh = thin_svd(Cs)
+        info.zero_()
+        check(lib().pamd_eig_from_svd_f64(n, _p(U), n, _p(Vh), n, _p(s), float(shift), _p(lamb), _p(info),
+                                          stream(dev)), "pamd_eig_from_svd_f64")
+        return Vh, s
+
+    Vh, s = run(0.0)
+    if int(info.item()) == 2:
+        Vh, s = run(float(s[0].item()))
+        order = torch.argsort(-lamb.abs(), stable=True)   # decreasing lambda now: by |lambda|
+        return lamb[order].contiguous(), transpose(Vh)[:, order].contiguous()
+    return lamb, transpose(Vh)
 
 
 def svd_right_precond(A_ske):

@@ -250,6 +250,35 @@ def test_sjlt_right_wide_rows_cross_lds_tiles(ops):
     assert rel(out, A @ S) < 100 * EPS
 
 
+def _fma_chain(terms):
+    """Correctly rounded fma(v, a, acc) chain from acc = 0, in the given order."""
+    from fractions import Fraction
+    acc = 0.0
+    for v, a in terms:
+        acc = float(Fraction(v) * Fraction(a) + Fraction(acc))
+    return acc
+
+
+@pytest.mark.parametrize("k,n,m", [(100, 600, 45), (300, 1100, 33), (600, 700, 20)])
+def test_sjlt_right_sums_in_column_order_with_fma(ops, k, n, m):
+    """Both right-product kernels (lane per row for k <= 512, thread per output
+    column beyond) give every output the fma chain over its CSR entries in
+    ascending column order, bit for bit."""
+    _say(f"sjlt right exact order k={k} n={n} m={m}")
+    import oracle as orc
+    rows, vals = orc.sjlt_draw(k, n, np.random.default_rng(k + n), 8)
+    rowptr, cols, cv = ops.sjlt_csr(k, n, 8, dev(rows.astype(np.int32)), dev(vals))
+    A = np.random.default_rng(m).standard_normal((m, n))
+    out = host(ops.sjlt_right(rowptr, cols, cv, k, dev(A)))
+    rp, cc, vv = host(rowptr), host(cols), host(cv)
+    rng = np.random.default_rng(1)
+    for r, c in zip(rng.integers(0, m, 60), rng.integers(0, k, 60)):
+        p0, p1 = rp[c], rp[c + 1]
+        assert np.all(np.diff(cc[p0:p1]) > 0)
+        want = _fma_chain([(vv[p], A[r, cc[p]]) for p in range(p0, p1)])
+        assert out[r, c] == want, (r, c, out[r, c], want)
+
+
 def test_device_sjlt_draw_invariants(ops):
     _say("sjlt device draw")
     d = torch.device("cuda")

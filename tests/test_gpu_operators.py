@@ -239,5 +239,10 @@ def test_spo_wide_problems(pa, n):
     x, log = pa.SPO(pa.SkOpSJ(8), 3)(A, b, 0.0, 1e-12, 60, np.random.default_rng(1))
     xl = la.lstsq(A, b)[0]
     assert la.norm(x - xl) <= 1e-8 * la.norm(xl)
-    assert orc.backward_error(A, b, x) <= 1e-12
-    assert log.errors.size <= 40
+    # the oracle on the same draws: same iteration count (to rounding) and a
+    # backward error as small as its own (4.4e-10 at n = 1901; LAPACK's gelsd
+    # lands at 4.3e-9 on this problem)
+    xo, logo = orc.spo(A, b, 0.0, 1e-12, 60, np.random.default_rng(1), sketch="sjlt", sampling_factor=3)
+    assert la.norm(x - xo) <= 1e-8 * la.norm(xo)
+    assert abs(log.errors.size - logo.errors.size) <= 2
+    assert orc.backward_error(A, b, x) <= 2 * orc.backward_error(A, b, xo)
