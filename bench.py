@@ -119,6 +119,9 @@ def main():
     ap.add_argument("--profile-batched", action="store_true",
                     help="only run the batched solve of the 512 uneven problems twice (the workload of the batched "
                          "kernel trace)")
+    ap.add_argument("--profile-learned", action="store_true",
+                    help="only train on --train-matrices matrices and run 4 learned-sketch and 4 plain SJLT solves "
+                         "on A 131072 x 512 (the workload of the learned-sketch kernel trace)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
     if args.gpus != 1:
@@ -167,6 +170,25 @@ def main():
             spo(Al, bl, 0.0, 1e-12, 100, i, logging=False)
         torch.cuda.synchronize()
         log("profile-ls: 12 plain FP64 SPO solves done")
+        return
+    if args.profile_learned:
+        import parla_amd.comps.sketchers.learned as lm
+        m5, n5, d5 = 131072, 512, 4096
+        data = lm.training_set(args.train_matrices, m5, n5, 50000, dev, log=log)
+        net = lm.LeverageMLP(seed=0)
+        net.fit(data, epochs=4, lr=2e-3, batch_rows=65536, log=log)
+        del data
+        A5, _ = lm.family_matrix(m5, n5, 777, dev)
+        rng5 = np.random.default_rng(5)
+        b5 = A5 @ torch.from_numpy(rng5.standard_normal(n5)).to(dev) + \
+            1e-6 * torch.from_numpy(rng5.standard_normal(m5)).to(dev)
+        learned = lm.SPOLearned(net, d=d5, kappa_max=30.0)
+        plain5 = pa.SPO(pa.SkOpSJ(8, backend="device"), 4)
+        for i in range(4):
+            learned(A5, b5, 0.0, 1e-12, 200, i, logging=False)
+            plain5(A5, b5, 0.0, 1e-12, 200, i, logging=False)
+        torch.cuda.synchronize()
+        log("profile-learned: 4 learned and 4 plain SJLT solves done")
         return
     if args.profile_batched:
         from parla_amd.drivers import batched_ls as bl_
@@ -458,15 +480,21 @@ def main():
         idx = np.random.default_rng(3).choice(P4, n_s4, replace=False)
         A4d = {int(i): As4[i].double() for i in idx}
         plain4 = pa.SPO(pa.SkOpSJ(8, backend="device"), 4)
-        tol4, etas_p4 = None, None
-        for cand in (1e-10, 3e-11, 1e-11, 3e-12, 1e-12):
+        # plain FP64 sketch-and-precondition reaches a backward error of about u kappa(A): on the
+        # worst-conditioned problems (kappa up to 1e10) no tolerance gets it to 1e-10, so the
+        # tolerance kept is the loosest that reaches the threshold on every sampled problem, else
+        # the one that reaches it on the most, and the count is reported
+        tol4, etas_p4, best = None, None, -1
+        for cand in (1e-10, 1e-11, 1e-12, 1e-13, 1e-14):
             log(f"C4: plain FP64 SPO on {n_s4} sampled problems at LSQR tol {cand:g}")
             e_c = []
             for i in idx:
                 xi, _ = plain4(A4d[int(i)], bs4[i], 0.0, cand, 400, int(i), logging=False)
                 e_c.append(device_eta(ops, torch, A4d[int(i)], bs4[i], xi))
-            tol4, etas_p4 = cand, e_c
-            if max(e_c) <= 1e-10:
+            meet = int(np.sum(np.array(e_c) <= 1e-10))
+            if meet > best:
+                tol4, etas_p4, best = cand, e_c, meet
+            if meet == n_s4:
                 break
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -499,9 +527,10 @@ def main():
             "batched_eta_extended_sample": [float(e) for e in etas],
             "batched_sum_mn": int(work),
             "plain_f64_one_at_a_time_solves_per_s": round(1.0 / t_p4, 2),
-            "plain_f64_sample": f"{n_s4} problems, SJLT SPO at LSQR tol {tol4:g} (the loosest of 1e-10 ... 1e-12 "
-                                "at which every sampled eta <= 1e-10)",
+            "plain_f64_sample": f"{n_s4} problems, SJLT SPO at LSQR tol {tol4:g} (the loosest of 1e-10 ... 1e-14 "
+                                "at which every sampled eta <= 1e-10, else the one meeting it most often)",
             "plain_f64_lsqr_tol": tol4, "plain_f64_max_eta": float(max(etas_p4)),
+            "plain_f64_meet_threshold": f"{best} of {n_s4}",
             "plain_f64_etas": [float(e) for e in etas_p4],
             "batched_speedup_vs_plain_f64": round((P4 / t_b4) * t_p4, 2),
             "batched_roofline": b4_roof,
