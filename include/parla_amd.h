@@ -19,7 +19,9 @@
  *  - Return value: 0 = ok; < 0 = argument -value is invalid (LAPACK info
  *    style); PAMD_ERR_* = runtime failure; > 0 = numerical event.
  *  - Entry points never abort and never throw across the ABI; they are
- *    re-entrant (no hidden global state besides the HIP runtime).
+ *    re-entrant (no hidden global state besides the HIP runtime), except
+ *    pamd_svd_jacobi_noalloc_f64, whose per-GPU static work array makes calls
+ *    that overlap on one GPU race.
  */
 #ifndef PARLA_AMD_H
 #define PARLA_AMD_H
@@ -216,6 +218,19 @@ int pamd_svd_jacobi_workspace(int64_t m, int64_t n, int64_t* bytes);
 int pamd_svd_jacobi_f64(int64_t m, int64_t n, const double* A, int64_t lda,
                         double* U, int64_t ldu, double* s, double* Vt, int64_t ldvt,
                         int max_sweeps, void* workspace, int* info, void* stream);
+
+/* pamd_svd_jacobi_f64 without a workspace argument and without allocating:
+ * the kernels work in a file-scope device array of the library (16.8 MB per
+ * GPU, held until the process ends), so m <= 1024 is the limit (larger m
+ * return -1).  No counterpart in the reference: this entry point is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other streams,
+ * other threads) race; the caller serialises them.  Same results, bit for
+ * bit, as pamd_svd_jacobi_f64. */
+int pamd_svd_jacobi_noalloc_f64(int64_t m, int64_t n, const double* A, int64_t lda,
+                                double* U, int64_t ldu, double* s, double* Vt,
+                                int64_t ldvt, int max_sweeps, int* info, void* stream);
 
 /* out (n x n) = C + shift * I for the symmetric C whose lower triangle A
  * holds (the upper triangle of A is not read, as scipy.linalg.eigh).  First

@@ -59,6 +59,8 @@ SIGNATURES = {
     "pamd_svd_jacobi_workspace": (c_int, [c_i64, c_i64, ctypes.POINTER(c_i64)]),
     "pamd_svd_jacobi_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_int, c_vp, c_vp,
                                     c_vp]),
+    "pamd_svd_jacobi_noalloc_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_int, c_vp,
+                                            c_vp]),
     "pamd_sym_shift_f64": (c_int, [c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp]),
     "pamd_eig_from_svd_f64": (c_int, [c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_dbl, c_vp, c_vp, c_vp]),
     "pamd_scale_cols_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_int, c_vp, c_i64, c_vp]),

@@ -41,6 +41,8 @@
 // W + j*m), V as np columns of np doubles, then the column norms, the sorted
 // positions and the per-sweep counters.  w = 16 for m <= 512, w = 8 for
 // m <= 1024 (the 2w LDS columns then take 128 KiB).
+#include <mutex>
+
 #include "common.h"
 
 namespace {
@@ -341,6 +343,77 @@ extern "C" int pamd_svd_jacobi_workspace(int64_t m, int64_t n, int64_t* bytes) {
   return PAMD_OK;
 }
 
+namespace {
+
+// the whole SVD on a caller-supplied or static workspace ws (arguments checked)
+int jac_svd(int64_t m, int64_t n, const double* A, int64_t lda, double* U, int64_t ldu, double* s, double* Vt,
+            int64_t ldvt, int max_sweeps, double* ws, int* info, hipStream_t st) {
+  const int w = jac_block(m);
+  const int64_t np = jac_nblocks(n, w) * w;
+  dim3 grid((unsigned)imin64(pamd_cdiv(imax64(m, np), 256), 4), (unsigned)np);
+  jac_init_kernel<<<grid, 256, 0, st>>>(m, n, np, A, lda, ws);
+  PAMD_LAUNCH_CHECK();
+  const int rc = jac_launch_sweeps(m, n, max_sweeps, ws, st);
+  if (rc != PAMD_OK) return rc;
+  return jac_finish(m, n, max_sweeps, U, ldu, s, Vt, ldvt, ws, info, st);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// The allocation-free variant: the same kernels on a file-scope device array
+// of JAC_SCRATCH_DOUBLES doubles (16.8 MB, the workspace of the largest SVD
+// this file does, m = n = 1024).  Each GPU holds its own copy of the array
+// until the process ends; its device address is looked up once per device
+// (hipGetSymbolAddress with that device current) and used for every launch --
+// the array's name is only a host-side stand-in.  Every call on one GPU
+// shares that GPU's array: calls that overlap on one GPU race.  Nothing in it
+// needs clearing between calls: the init kernel writes W, V and the sweep
+// counters, and the norms and positions are written before they are read.
+// ---------------------------------------------------------------------------
+constexpr int64_t JAC_SCRATCH_DOUBLES = 2 * JAC_MAX_M * JAC_MAX_M + 4 * JAC_MAX_M + 256;
+static __device__ __align__(256) double scratch[JAC_SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int JAC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[JAC_MAX_DEVICES] = {};
+
+// device address of the current device's copy of `scratch`
+int jac_scratch(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= JAC_MAX_DEVICES) return PAMD_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return PAMD_ERR_HIP;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PAMD_OK;
+}
+
+}  // namespace
+
+extern "C" int pamd_svd_jacobi_noalloc_f64(int64_t m, int64_t n, const double* A, int64_t lda, double* U,
+                                           int64_t ldu, double* s, double* Vt, int64_t ldvt, int max_sweeps,
+                                           int* info, void* stream) {
+  PAMD_ARG_CHECK(m >= 1 && m <= JAC_MAX_M, 1);
+  PAMD_ARG_CHECK(n >= 1 && n <= m, 2);
+  PAMD_ARG_CHECK(A != nullptr, 3);
+  PAMD_ARG_CHECK(lda >= n, 4);
+  PAMD_ARG_CHECK(U == nullptr || ldu >= n, 6);
+  PAMD_ARG_CHECK(s != nullptr, 7);
+  PAMD_ARG_CHECK(Vt == nullptr || ldvt >= n, 9);
+  PAMD_ARG_CHECK(max_sweeps >= 1 && max_sweeps <= JAC_MAX_SWEEPS, 10);
+  if (jac_layout(m, n).total > JAC_SCRATCH_DOUBLES) return PAMD_ERR_UNSUPPORTED;
+  double* ws = nullptr;
+  const int rc = jac_scratch(&ws);
+  if (rc != PAMD_OK) return rc;
+  return jac_svd(m, n, A, lda, U, ldu, s, Vt, ldvt, max_sweeps, ws, info, (hipStream_t)stream);
+}
+
 extern "C" int pamd_svd_jacobi_f64(int64_t m, int64_t n, const double* A, int64_t lda, double* U, int64_t ldu,
                                    double* s, double* Vt, int64_t ldvt, int max_sweeps, void* workspace, int* info,
                                    void* stream) {
@@ -353,18 +426,8 @@ extern "C" int pamd_svd_jacobi_f64(int64_t m, int64_t n, const double* A, int64_
   PAMD_ARG_CHECK(Vt == nullptr || ldvt >= n, 9);
   PAMD_ARG_CHECK(max_sweeps >= 1 && max_sweeps <= JAC_MAX_SWEEPS, 10);
   PAMD_ARG_CHECK(workspace != nullptr, 11);
-  hipStream_t st = (hipStream_t)stream;
-  double* ws = (double*)workspace;
-  const int w = jac_block(m);
-  const int64_t np = jac_nblocks(n, w) * w;
-  dim3 grid((unsigned)imin64(pamd_cdiv(imax64(m, np), 256), 4), (unsigned)np);
-  jac_init_kernel<<<grid, 256, 0, st>>>(m, n, np, A, lda, ws);
-  PAMD_LAUNCH_CHECK();
-  const int rc = jac_launch_sweeps(m, n, max_sweeps, ws, st);
-  if (rc != PAMD_OK) return rc;
-  return jac_finish(m, n, max_sweeps, U, ldu, s, Vt, ldvt, ws, info, st);
+  return jac_svd(m, n, A, lda, U, ldu, s, Vt, ldvt, max_sweeps, (double*)workspace, info, (hipStream_t)stream);
 }
-
 
 namespace {
 
