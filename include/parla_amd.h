@@ -468,6 +468,19 @@ int pamd_row_scatter_f64(int64_t d, int64_t m, int64_t n, const int64_t* idx,
                          const double* A, int64_t lda, double* out, int64_t ldo,
                          void* stream);
 
+/* Workspace (bytes) for pamd_sample_rows_f64 (the CDF: m doubles). */
+int pamd_sample_rows_workspace(int64_t m, int64_t* bytes);
+
+/* d row indices drawn with replacement from the weights p (m >= 1, p_i >= 0,
+ * sum > 0; need not be normalised) by inverse CDF on Philox-4x32-10 uniforms
+ * (counter = draw index, key = seed), and the scales 1 / sqrt(d p_idx / sum p)
+ * of the sampling sketch.  The CDF is summed in a fixed order, so the draws
+ * depend only on (p, d, seed).  The weighted row-sampling counterpart of
+ * sampling_operator (utils/sketching.py:204-232) for the learned sketch; no
+ * reference call site draws weighted rows. */
+int pamd_sample_rows_f64(int64_t m, const double* p, int64_t d, uint64_t seed,
+                         int64_t* idx, double* scale, void* workspace, void* stream);
+
 /* Batched potrf / trtri: problem i is the n[i] x n[i] matrix (ld n[i]) at
  * element offset off[i] of the pool (n and off are device arrays; n[i] <= 512
  * for potrf, nmax = max n[i]).  One workgroup per problem (per column for

@@ -118,6 +118,8 @@ SIGNATURES = {
     "pamd_mlp_adam": (c_int, [c_vp, c_vp, c_vp, c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                               ctypes.c_float, c_i64, c_vp, c_vp, c_vp]),
     "pamd_row_gather_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp]),
+    "pamd_sample_rows_workspace": (c_int, [c_i64, ctypes.POINTER(c_i64)]),
+    "pamd_sample_rows_f64": (c_int, [c_i64, c_vp, c_i64, c_u64, c_vp, c_vp, c_vp, c_vp]),
     "pamd_row_scatter_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp]),
     "pamd_lsqr_workspace": (c_int, [c_i64, c_i64, c_i64, ctypes.POINTER(c_i64)]),
     "pamd_lsqr_run_f64": (c_int, [ctypes.POINTER(LsqrProblem), c_vp, c_vp, c_vp, c_vp,

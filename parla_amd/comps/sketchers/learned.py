@@ -199,19 +199,33 @@ def sampling_probabilities(y, mix=0.1, n=None):
     return (1.0 - mix) * p + mix / y.numel()
 
 
-def sample_rows(p, d, rng):
-    """d row indices drawn with replacement from p (numpy Generator on the host)
-    and their scales 1/sqrt(d p_i)."""
-    ph = p.cpu().numpy()
-    ph = ph / ph.sum()
-    idx = rng.choice(ph.size, size=d, replace=True, p=ph)
-    return idx, 1.0 / np.sqrt(d * ph[idx])
+def sample_rows(p, d, rng, backend="device"):
+    """d row indices drawn with replacement from the weights p and their
+    scales 1/sqrt(d p_i / sum p).  backend 'device': inverse CDF on Philox
+    uniforms in the library (pamd_sample_rows_f64; seed from rng), results on
+    the device, no host round trip; 'numpy': numpy's Generator.choice on the
+    host, results as numpy arrays."""
+    if backend == "numpy":
+        ph = p.cpu().numpy()
+        ph = ph / ph.sum()
+        idx = rng.choice(ph.size, size=d, replace=True, p=ph)
+        return idx, 1.0 / np.sqrt(d * ph[idx])
+    dev = p.device
+    pc = p.to(torch.float64).contiguous()
+    m = pc.numel()
+    idx = torch.empty(d, dtype=torch.int64, device=dev)
+    scale = torch.empty(d, dtype=torch.float64, device=dev)
+    ws = ops.workspace(ops._query(ops.lib().pamd_sample_rows_workspace, m), dev)
+    ops.check(ops.lib().pamd_sample_rows_f64(m, ops._p(pc), d, int(rng.integers(0, 2 ** 63)), ops._p(idx),
+                                             ops._p(scale), ops._p(ws), ops.stream(dev)), "pamd_sample_rows_f64")
+    return idx, scale
 
 
 def row_gather(A, idx, scale, out):
     dev = A.device
-    it = torch.from_numpy(np.asarray(idx, dtype=np.int64)).to(dev)
-    st = torch.from_numpy(np.asarray(scale, dtype=np.float64)).to(dev)
+    it = idx if isinstance(idx, torch.Tensor) else torch.from_numpy(np.asarray(idx, dtype=np.int64)).to(dev)
+    st = scale if isinstance(scale, torch.Tensor) else \
+        torch.from_numpy(np.asarray(scale, dtype=np.float64)).to(dev)
     m, n = A.shape
     ops.check(ops.lib().pamd_row_gather_f64(len(idx), m, n, ops._p(it), ops._p(st), ops._p(A), A.stride(0),
                                             ops._p(out), out.stride(0), ops.stream(dev)), "pamd_row_gather_f64")
@@ -254,7 +268,7 @@ class SPOLearned:
     """SPO (least_squares.py:206-369 call contract) with the learned row-sampling
     sketch and the plain SJLT SPO as the fallback."""
 
-    def __init__(self, mlp, d=4096, kappa_max=10.0, mix=0.1, fallback=None, lanczos_steps=12):
+    def __init__(self, mlp, d=4096, kappa_max=10.0, mix=0.1, fallback=None, lanczos_steps=12, sampling="device"):
         from parla_amd.drivers.least_squares import SPO
         import parla_amd.comps.sketchers.oblivious as sko
         self.mlp = mlp
@@ -262,6 +276,7 @@ class SPOLearned:
         self.kappa_max = kappa_max
         self.mix = mix
         self.lanczos_steps = lanczos_steps
+        self.sampling = sampling
         self.fallback = fallback or SPO(sko.SkOpSJ(8), 4)
         self.iterative_solver = dsad.PcSS2()
 
@@ -272,10 +287,12 @@ class SPOLearned:
         if probs is None:
             G = presketch(n, int(rng.integers(0, 2 ** 63)), dev)
             probs = sampling_probabilities(self.mlp.predict(features(At, G)), self.mix, n)
-        idx, scale = sample_rows(probs, self.d, rng)
+        idx, scale = sample_rows(probs, self.d, rng, self.sampling)
         Mx = torch.empty((self.d, n + 1), dtype=ops.F64, device=dev)
         row_gather(At, idx, scale, Mx[:, :n])
-        Mx[:, n] = bt[torch.from_numpy(idx).to(dev)] * torch.from_numpy(scale).to(dev)
+        it = idx if isinstance(idx, torch.Tensor) else torch.from_numpy(idx).to(dev)
+        st = scale if isinstance(scale, torch.Tensor) else torch.from_numpy(scale).to(dev)
+        row_gather(bt.reshape(-1, 1), it, st, Mx[:, n:])
         fac = ops.geqrf_(Mx, n)
         R = torch.triu(fac.R_view).contiguous()
         z = Mx[:n, n].clone()

@@ -39,19 +39,23 @@
 //
 // Layout in the workspace: W as np columns of m doubles (column j at
 // W + j*m), V as np columns of np doubles, then the column norms, the sorted
-// positions and the per-sweep counters.  w = 16 for m <= 512, w = 8 for
-// m <= 1024 (the 2w LDS columns then take 128 KiB).
+// positions and the per-sweep counters.  w = 8: a round workgroup is eight
+// wavefronts, and its 16 LDS columns take 64 KiB (m <= 512) or 128 KiB
+// (m <= 1024).  Narrow blocks spread a sweep over more workgroups at once
+// (17 per round at n = 266) with fewer wavefronts per SIMD; the rotation's
+// reciprocal and square roots are hardware estimates refined by Newton steps.
 #include <mutex>
 
 #include "common.h"
 
 namespace {
 
-constexpr int JAC_THREADS = 1024;
+constexpr int JAC_BW = 8;                 // columns per block: 16 per pair, one wavefront per column pair
+constexpr int JAC_THREADS = 64 * JAC_BW;  // one round workgroup
 constexpr int JAC_MAX_SWEEPS = 64;
 constexpr int64_t JAC_MAX_M = 1024;
 
-__host__ __device__ inline int jac_block(int64_t m) { return m <= 512 ? 16 : 8; }
+__host__ __device__ inline int jac_block(int64_t) { return JAC_BW; }
 
 __host__ __device__ inline int64_t jac_nblocks(int64_t n, int w) {
   int64_t nb = pamd_cdiv(n, w);
@@ -105,12 +109,29 @@ __global__ void jac_init_kernel(int64_t m, int64_t n, int64_t np, const double* 
   if (j == 0 && blockIdx.x == 0 && threadIdx.x < JAC_MAX_SWEEPS) cnt[threadIdx.x] = 0;
 }
 
+// 1 / x and 1 / sqrt(x) from the hardware estimates and two Newton steps
+// each (the estimates' relative errors are squared twice: a few ulps)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = y * fma(-x, y, 2.0);
+  return y * fma(-x, y, 2.0);
+}
+
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  return y * fma(-h * y, y, 1.5);
+}
+
 // One round of sweep `sweep`: workgroup k orthogonalises block pair k.
+// MMAX >= m rows of the pair's columns are staged in LDS.
 template <int BW, int MMAX>
-__global__ __launch_bounds__(JAC_THREADS) void jac_round_kernel(int64_t m, int64_t n, int nb, int sweep, int round,
-                                                                double tol, double* __restrict__ ws) {
+__global__ __launch_bounds__(64 * BW) void jac_round_kernel(int64_t m, int64_t n, int nb, int sweep, int round,
+                                                            double tol, double* __restrict__ ws) {
   constexpr int P = 2 * BW;         // columns of the pair
   constexpr int E = MMAX / 64;      // rows per lane
+  constexpr int NT = 64 * BW;       // threads
   __shared__ double Ws[P][MMAX];    // the pair's columns of W, column-major
   __shared__ double Js[P][P + 1];   // accumulated rotation, Js[col][row]
   const JacLayout L = jac_layout(m, n);
@@ -118,7 +139,8 @@ __global__ __launch_bounds__(JAC_THREADS) void jac_round_kernel(int64_t m, int64
   if (sweep > 0 && cnt[sweep - 1] == 0) return;  // an earlier sweep rotated nothing: converged
   double* W = ws + L.W;
   double* V = ws + L.V;
-  const int64_t np = (int64_t)nb * BW;
+  const int np = nb * BW;
+  const int mi = (int)m;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   int bp, bq;
@@ -126,83 +148,86 @@ __global__ __launch_bounds__(JAC_THREADS) void jac_round_kernel(int64_t m, int64
   // global column of pair column c
   auto gcol = [&](int c) -> int64_t { return (c < BW) ? (int64_t)bp * BW + c : (int64_t)bq * BW + (c - BW); };
 
-  for (int64_t idx = tid; idx < (int64_t)P * m; idx += JAC_THREADS) {
-    const int c = (int)(idx / m);
-    const int64_t i = idx - (int64_t)c * m;
+  for (int idx = tid; idx < P * mi; idx += NT) {
+    const int c = idx / mi, i = idx - c * mi;
     Ws[c][i] = W[gcol(c) * m + i];
   }
-  for (int idx = tid; idx < P * P; idx += JAC_THREADS) Js[idx / P][idx % P] = (idx / P == idx % P) ? 1.0 : 0.0;
+  for (int idx = tid; idx < P * P; idx += NT) Js[idx / P][idx % P] = (idx / P == idx % P) ? 1.0 : 0.0;
   __syncthreads();
 
-  const int em = (int)pamd_cdiv(m, 64);
+  const int em = (mi + 63) / 64;
+  const double tol2 = tol * tol;
   int rotated = 0;
   for (int t = 0; t < P - 1; ++t) {
-    if (wave < BW) {
-      int a, b;
-      circle_pair(t, wave, P - 1, a, b);
-      double xa[E], xb[E];
-      double al = 0.0, be = 0.0, ga = 0.0;
+    int a, b;
+    circle_pair(t, wave, P - 1, a, b);
+    double xa[E], xb[E];
+    double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      xa[e] = 0.0;
+      xb[e] = 0.0;
+      const int i = lane + 64 * e;
+      if (e < em && i < mi) {
+        xa[e] = Ws[a][i];
+        xb[e] = Ws[b][i];
+      }
+      al = fma(xa[e], xa[e], al);
+      be = fma(xb[e], xb[e], be);
+      ga = fma(xa[e], xb[e], ga);
+    }
+    al = wave_sum(al);
+    be = wave_sum(be);
+    ga = wave_sum(ga);
+    // rotate unless |ga| <= tol sqrt(al be) (also skips zero columns)
+    if (ga * ga > tol2 * (al * be)) {
+      // a' = c a - s b, b' = s a + c b with a'.b' = 0: t = tan(theta) is the
+      // smaller root of t^2 + 2 zeta t - 1 = 0, zeta = (be - al) / (2 ga)
+      const double zeta = (be - al) * rcp_nr(2.0 * ga);
+      const double az = fabs(zeta);
+      double tt;
+      if (az > 1e150) {
+        tt = 0.5 * rcp_nr(zeta);
+      } else {
+        const double q = fma(zeta, zeta, 1.0);
+        tt = copysign(rcp_nr(az + q * rsq_nr(q)), zeta);
+      }
+      const double c = rsq_nr(fma(tt, tt, 1.0));
+      const double s = c * tt;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        xa[e] = 0.0;
-        xb[e] = 0.0;
-        const int64_t i = lane + 64 * e;
-        if (e < em && i < m) {
-          xa[e] = Ws[a][i];
-          xb[e] = Ws[b][i];
+        const int i = lane + 64 * e;
+        if (e < em && i < mi) {
+          Ws[a][i] = c * xa[e] - s * xb[e];
+          Ws[b][i] = s * xa[e] + c * xb[e];
         }
-        al = fma(xa[e], xa[e], al);
-        be = fma(xb[e], xb[e], be);
-        ga = fma(xa[e], xb[e], ga);
       }
-      al = wave_sum(al);
-      be = wave_sum(be);
-      ga = wave_sum(ga);
-      if (fabs(ga) > tol * sqrt(al) * sqrt(be)) {
-        // a' = c a - s b, b' = s a + c b with a'.b' = 0: t = tan(theta) is the
-        // smaller root of t^2 + 2 zeta t - 1 = 0, zeta = (be - al) / (2 ga)
-        const double zeta = (be - al) / (2.0 * ga);
-        const double az = fabs(zeta);
-        const double tt = (az > 1e150) ? 0.5 / zeta : copysign(1.0 / (az + sqrt(1.0 + zeta * zeta)), zeta);
-        const double c = 1.0 / sqrt(1.0 + tt * tt);
-        const double s = c * tt;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const int64_t i = lane + 64 * e;
-          if (e < em && i < m) {
-            Ws[a][i] = c * xa[e] - s * xb[e];
-            Ws[b][i] = s * xa[e] + c * xb[e];
-          }
-        }
-        if (lane < P) {
-          const double ja = Js[a][lane], jb = Js[b][lane];
-          Js[a][lane] = c * ja - s * jb;
-          Js[b][lane] = s * ja + c * jb;
-        }
-        rotated = 1;
+      if (lane < P) {
+        const double ja = Js[a][lane], jb = Js[b][lane];
+        Js[a][lane] = c * ja - s * jb;
+        Js[b][lane] = s * ja + c * jb;
       }
+      rotated = 1;
     }
     __syncthreads();
   }
   const int any = __syncthreads_or(rotated);
   if (!any) return;  // the pair was already orthogonal: W and V unchanged
 
-  for (int64_t idx = tid; idx < (int64_t)P * m; idx += JAC_THREADS) {
-    const int c = (int)(idx / m);
-    const int64_t i = idx - (int64_t)c * m;
+  for (int idx = tid; idx < P * mi; idx += NT) {
+    const int c = idx / mi, i = idx - c * mi;
     W[gcol(c) * m + i] = Ws[c][i];
   }
   __syncthreads();
   // V(:, pair) <- V(:, pair) J: the pair's V columns (np <= MMAX rows) are
   // staged in the LDS the W columns used, then each thread forms one row's
   // 2w outputs, eight at a time
-  for (int64_t idx = tid; idx < (int64_t)P * np; idx += JAC_THREADS) {
-    const int c = (int)(idx / np);
-    const int64_t i = idx - (int64_t)c * np;
+  for (int idx = tid; idx < P * np; idx += NT) {
+    const int c = idx / np, i = idx - c * np;
     Ws[c][i] = V[gcol(c) * np + i];
   }
   __syncthreads();
-  for (int64_t i = tid; i < np; i += JAC_THREADS) {
+  for (int i = tid; i < np; i += NT) {
 #pragma unroll
     for (int j0 = 0; j0 < P; j0 += 8) {
       double acc[8];
@@ -309,10 +334,10 @@ int jac_launch_sweeps(int64_t m, int64_t n, int max_sweeps, double* ws, hipStrea
   const double tol = (double)imax64(m, 16) * 2.220446049250313e-16;
   for (int sw = 0; sw < max_sweeps; ++sw) {
     for (int r = 0; r < nb - 1; ++r) {
-      if (w == 16)
-        jac_round_kernel<16, 512><<<nb / 2, JAC_THREADS, 0, st>>>(m, n, nb, sw, r, tol, ws);
+      if (m <= 512)
+        jac_round_kernel<JAC_BW, 512><<<nb / 2, JAC_THREADS, 0, st>>>(m, n, nb, sw, r, tol, ws);
       else
-        jac_round_kernel<8, 1024><<<nb / 2, JAC_THREADS, 0, st>>>(m, n, nb, sw, r, tol, ws);
+        jac_round_kernel<JAC_BW, 1024><<<nb / 2, JAC_THREADS, 0, st>>>(m, n, nb, sw, r, tol, ws);
       PAMD_LAUNCH_CHECK();
     }
   }

@@ -18,6 +18,7 @@
 // workgroup partials are summed in a fixed order by a second kernel, so the
 // gradients are deterministic.
 #include "common.h"
+#include "rng.h"
 
 namespace {
 
@@ -357,6 +358,90 @@ extern "C" int pamd_row_scatter_f64(int64_t d, int64_t m, int64_t n, const int64
   PAMD_ARG_CHECK(ldo >= n, 8);
   if (d == 0 || n == 0) return PAMD_OK;
   row_scatter_kernel<<<(unsigned)d, 256, 0, (hipStream_t)stream>>>(n, idx, A, lda, out, ldo);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+namespace {
+
+// Inclusive prefix sums of the m row weights p in one 1024-thread workgroup:
+// thread t sums its contiguous chunk in order, thread 0 scans the 1024 chunk
+// totals in order, then each thread writes its chunk's running sums.  A fixed
+// summation order, so the CDF is run-to-run identical.
+__global__ __launch_bounds__(1024) void weights_cdf_kernel(int64_t m, const double* __restrict__ p,
+                                                           double* __restrict__ cdf) {
+  __shared__ double tot[1024];
+  const int t = threadIdx.x;
+  const int64_t chunk = pamd_cdiv(m, 1024);
+  const int64_t i0 = imin64(m, (int64_t)t * chunk), i1 = imin64(m, i0 + chunk);
+  double s = 0.0;
+  for (int64_t i = i0; i < i1; ++i) s += p[i];
+  tot[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    double run = 0.0;
+    for (int k = 0; k < 1024; ++k) {
+      const double v = tot[k];
+      tot[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  double run = tot[t];
+  for (int64_t i = i0; i < i1; ++i) {
+    run += p[i];
+    cdf[i] = run;
+  }
+}
+
+// Draw i: u in (0, 1] from Philox-4x32-10 at counter i, the row j with
+// cdf[j - 1] < u * total <= cdf[j] (binary search; a row of weight 0 is never
+// chosen since u > 0), and the scale 1 / sqrt(d p_j / total) of a sampling
+// sketch with replacement.
+__global__ void sample_rows_kernel(int64_t m, int64_t d, uint64_t seed, const double* __restrict__ p,
+                                   const double* __restrict__ cdf, int64_t* __restrict__ idx,
+                                   double* __restrict__ scale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d) return;
+  const U4 r = philox4x32_10(U4{(uint32_t)i, (uint32_t)(i >> 32), 0x3C6EF372u, 0x510E527Fu}, (uint32_t)seed,
+                             (uint32_t)(seed >> 32));
+  const double total = cdf[m - 1];
+  const double target = u53_open0(r.x, r.y) * total;
+  int64_t lo = 0, hi = m - 1;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (cdf[mid] >= target)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  idx[i] = lo;
+  scale[i] = 1.0 / sqrt((double)d * p[lo] / total);
+}
+
+}  // namespace
+
+extern "C" int pamd_sample_rows_workspace(int64_t m, int64_t* bytes) {
+  PAMD_ARG_CHECK(m >= 1, 1);
+  PAMD_ARG_CHECK(bytes != nullptr, 2);
+  *bytes = m * (int64_t)sizeof(double);
+  return PAMD_OK;
+}
+
+extern "C" int pamd_sample_rows_f64(int64_t m, const double* p, int64_t d, uint64_t seed, int64_t* idx,
+                                    double* scale, void* workspace, void* stream) {
+  PAMD_ARG_CHECK(m >= 1, 1);
+  PAMD_ARG_CHECK(p != nullptr, 2);
+  PAMD_ARG_CHECK(d >= 0, 3);
+  PAMD_ARG_CHECK(idx != nullptr || d == 0, 5);
+  PAMD_ARG_CHECK(scale != nullptr || d == 0, 6);
+  PAMD_ARG_CHECK(workspace != nullptr, 7);
+  if (d == 0) return PAMD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  double* cdf = (double*)workspace;
+  weights_cdf_kernel<<<1, 1024, 0, st>>>(m, p, cdf);
+  PAMD_LAUNCH_CHECK();
+  sample_rows_kernel<<<(unsigned)pamd_cdiv(d, 256), 256, 0, st>>>(m, d, seed, p, cdf, idx, scale);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

@@ -7,7 +7,7 @@
 // atomics:
 //   left   out[i, :] = sum_{(j, v) in row i of W} v * A[j, :]     (W @ A)
 //   right  out[r, c] = sum_{(j, v) in row c of W} v * A[r, j]     (A @ W^T)
-// The right product (k <= 512) stages 32-row tiles of A in LDS, transposed,
+// The right product (k <= 288) stages 32-row tiles of A in LDS, transposed,
 // with one lane per row and each output column's CSR list walked by a
 // half-wavefront, so W's index arrays are read once per 32 rows of A; wider
 // outputs use one thread per output column over 16-row tiles.
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(1024) void sjlt_right_kernel(int64_t m, int64_t n, 
       if (rr < nr) out[(r0 + rr) * ldo + c] = acc[rr];
 }
 
-// A * W^T for k <= 512 output columns, one lane per row of A.  A workgroup of
+// A * W^T for k <= 288 output columns, one lane per row of A.  A workgroup of
 // 1024 threads owns RS_ROWS = 32 rows of A and every output column: each
 // half-wavefront (32 lanes, lane = row) owns the output columns c = h, h + 32,
 // h + 64, ... and keeps their accumulators and CSR cursors in registers.  A
@@ -264,32 +264,39 @@ __global__ __launch_bounds__(RS_THREADS) void sjlt_right_rows_kernel(int64_t m, 
       As[e % RS_TJ][e / RS_TJ] = pre[t];
     }
     __syncthreads();
-    if (j0 + RS_TJ < n) load_tile(j0 + RS_TJ);  // in flight while this tile is used
     const int j1 = (int)imin64(n, j0 + RS_TJ);
+    // the entries of column q in [j0, j1) are a prefix of what remains
+    // (sorted by j): the 32 lanes of the half-wave fetch the next 32 entries
+    // of every column at once (one memory round trip per tile), the ballot
+    // counts those inside this tile, and the lanes read them in order from
+    // the fetching lanes.  They are issued before the next tile's loads:
+    // loads complete in order, so waiting for them then does not wait for
+    // the next tile.
+    int jv[CPH];
+    double vv[CPH];
 #pragma unroll
     for (int q = 0; q < CPH; ++q) {
-      // the column's entries in [j0, j1) are a prefix of what remains (sorted
-      // by j); fetched four at a time, so the index loads are waited on once
-      // per four entries
-      while (p[q] < pe[q]) {
-        int jv[4];
-        double vv[4];
+      const bool in = p[q] + r < pe[q];
+      jv[q] = in ? cols[p[q] + r] : INT32_MAX;
+      vv[q] = in ? vals[p[q] + r] : 0.0;
+    }
+    if (j0 + RS_TJ < n) load_tile(j0 + RS_TJ);  // in flight while this tile is used
+    const int hb = (tid & 63) & 32;  // first lane of this half-wave
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const bool in = p[q] + e < pe[q];
-          jv[e] = in ? cols[p[q] + e] : INT32_MAX;
-          vv[e] = in ? vals[p[q] + e] : 0.0;
+    for (int q = 0; q < CPH; ++q) {
+      for (;;) {
+        const uint64_t mask = __ballot(jv[q] < j1);
+        const int cnt = __popcll((mask >> hb) & 0xffffffffull);
+        for (int e = 0; e < cnt; ++e) {
+          const int je = __shfl(jv[q], hb + e, 64);
+          const double ve = __shfl(vv[q], hb + e, 64);
+          acc[q] = fma(ve, As[je - (int)j0][r], acc[q]);
         }
-        int used = 0;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if (jv[e] < j1) {
-            acc[q] = fma(vv[e], As[jv[e] - (int)j0][r], acc[q]);
-            used = e + 1;
-          }
-        }
-        p[q] += used;
-        if (used < 4) break;  // the rest of this column lies in later tiles
+        p[q] += cnt;
+        if (cnt < 32) break;  // the rest of this column lies in later tiles
+        const bool in = p[q] + r < pe[q];
+        jv[q] = in ? cols[p[q] + r] : INT32_MAX;
+        vv[q] = in ? vals[p[q] + r] : 0.0;
       }
     }
   }
@@ -389,7 +396,7 @@ extern "C" int pamd_sjlt_csr(int64_t n_short, int64_t n_long, int k, const int32
 
 namespace {
 
-// launch sjlt_right_rows_kernel<cph> (cph = ceil(k / 32) <= 16); false if not built for cph
+// launch sjlt_right_rows_kernel<cph> (cph = ceil(k / 32) <= 9: beyond, its registers spill); false otherwise
 bool sjlt_right_rows_launch(int cph, int64_t m, int64_t n, int64_t k, const int64_t* rowptr, const int32_t* cols,
                             const double* vals, const double* A, int64_t lda, double* out, int64_t ldo,
                             hipStream_t st) {
@@ -399,8 +406,7 @@ bool sjlt_right_rows_launch(int cph, int64_t m, int64_t n, int64_t k, const int6
     sjlt_right_rows_kernel<C><<<nb, RS_THREADS, 0, st>>>(m, n, k, rowptr, cols, vals, A, lda, out, ldo); \
     return true;
   switch (cph) {
-    PAMD_RS(1) PAMD_RS(2) PAMD_RS(3) PAMD_RS(4) PAMD_RS(5) PAMD_RS(6) PAMD_RS(7) PAMD_RS(8)
-    PAMD_RS(9) PAMD_RS(10) PAMD_RS(11) PAMD_RS(12) PAMD_RS(13) PAMD_RS(14) PAMD_RS(15) PAMD_RS(16)
+    PAMD_RS(1) PAMD_RS(2) PAMD_RS(3) PAMD_RS(4) PAMD_RS(5) PAMD_RS(6) PAMD_RS(7) PAMD_RS(8) PAMD_RS(9)
     default:
       return false;
   }
@@ -441,7 +447,7 @@ extern "C" int pamd_sjlt_right_f64(int64_t m, int64_t n, int64_t k, const int64_
   if (m == 0 || k == 0) return PAMD_OK;
   hipStream_t st = (hipStream_t)stream;
   const int cph = (int)pamd_cdiv(k, RS_HALVES);
-  if (cph <= 16 && k * n < (1LL << 31) && sjlt_right_rows_launch(cph, m, n, k, rowptr, cols, vals, A, lda, out, ldo,
+  if (cph <= 9 && k * n < (1LL << 31) && sjlt_right_rows_launch(cph, m, n, k, rowptr, cols, vals, A, lda, out, ldo,
                                                                 st)) {
     PAMD_LAUNCH_CHECK();
     return PAMD_OK;
