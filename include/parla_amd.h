@@ -345,6 +345,16 @@ int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, double* x_orig,
                       double* var, double* arnorms, pamd_lsqr_result* result,
                       void* workspace, void* stream);
 
+/* The same LSQR (comps/determiter/lsqr.py:98-574 on the operator of
+ * pamd_lsqr_run_f64) as launches only: no host synchronisation and no
+ * read-back, so a caller can capture it in a HIP graph.  All iter_lim
+ * iterations are launched; those after the stopping test fired return at
+ * once.  *result is DEVICE memory (sizeof(pamd_lsqr_result) bytes), written
+ * by the last launch.  Results equal pamd_lsqr_run_f64's bit for bit. */
+int pamd_lsqr_launch_f64(const pamd_lsqr_problem* prob, double* x, double* x_orig,
+                         double* var, double* arnorms, pamd_lsqr_result* result,
+                         void* workspace, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Mixed precision: exact BF16 sketch, compensated residual, CG operator      */
 /* ------------------------------------------------------------------------ */

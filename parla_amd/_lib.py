@@ -124,6 +124,7 @@ SIGNATURES = {
     "pamd_lsqr_workspace": (c_int, [c_i64, c_i64, c_i64, ctypes.POINTER(c_i64)]),
     "pamd_lsqr_run_f64": (c_int, [ctypes.POINTER(LsqrProblem), c_vp, c_vp, c_vp, c_vp,
                                   ctypes.POINTER(LsqrResult), c_vp, c_vp]),
+    "pamd_lsqr_launch_f64": (c_int, [ctypes.POINTER(LsqrProblem), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
 }
 
 _lib = None

@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
                                                        double alpha, const double* __restrict__ A, int64_t lda,
                                                        const double* __restrict__ B, int64_t ldb, double beta,
                                                        double* __restrict__ C, int64_t ldc,
-                                                       double* __restrict__ part, int* __restrict__ cnt) {
+                                                       double* __restrict__ part) {
   constexpr int IM = BM / 32, JN = BN / 32;  // MFMA blocks per wavefront
   __shared__ double As[GBK][BM + 1];
   __shared__ double Bs[GBK][BN + 1];
@@ -190,50 +190,6 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
           }
         }
       }
-  if (part == nullptr || cnt == nullptr) return;
-  // Split K, fused reduction: the last slice of this tile to finish sums the
-  // tile's slice partials in slice order (the order of splitk_reduce_kernel,
-  // so the bits do not depend on which slice finishes last) and writes C.
-  // The release fence makes this slice's partial visible device-wide before
-  // the tile's counter counts it; the acquire fence of the last slice makes
-  // the others' visible to it.  The last slice resets the counter for the
-  // next call.
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int prev = atomicAdd(cnt + t2, 1);
-    last = (prev == (int)gridDim.z - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const int splits = (int)gridDim.z;
-  const int64_t mn = M * N;
-  for (int e = threadIdx.x; e < BM * BN; e += blockDim.x) {
-    const int64_t row = m0 + e / BN, col = n0 + e % BN;
-    if (row >= M || col >= N) continue;
-    const double* pp = part + row * N + col;
-    double sum = 0.0;
-    int z = 0;
-    for (; z + 4 <= splits; z += 4) {
-      const double p0 = pp[(int64_t)z * mn], p1 = pp[(int64_t)(z + 1) * mn];
-      const double p2 = pp[(int64_t)(z + 2) * mn], p3 = pp[(int64_t)(z + 3) * mn];
-      sum += p0;
-      sum += p1;
-      sum += p2;
-      sum += p3;
-    }
-    for (; z < splits; ++z) sum += pp[(int64_t)z * mn];
-    double* c = C + row * ldc + col;
-    *c = (beta == 0.0) ? alpha * sum : alpha * sum + beta * (*c);
-  }
-  if (threadIdx.x == 0) cnt[t2] = 0;
-}
-
-__global__ void zero_ints_kernel(int* __restrict__ p, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
 }
 
 // Tile edge for a dimension of size d: 96 when that pads at least 5% less
@@ -394,16 +350,10 @@ __global__ __launch_bounds__(1024) void trsv_global_kernel(int trans, int n, con
 
 }  // namespace
 
-// bytes of the slice partials and of the per-tile counters of the fused reduction
-inline int64_t gemm_part_bytes(int64_t M, int64_t N, int splits) { return (int64_t)splits * M * N * sizeof(double); }
-inline int64_t gemm_cnt_bytes(int64_t M, int64_t N) {
-  return pamd_cdiv(M, gemm_edge(M)) * pamd_cdiv(N, gemm_edge(N)) * (int64_t)sizeof(int);
-}
-
 extern "C" int pamd_gemm_workspace(int64_t M, int64_t N, int64_t K, int64_t* bytes) {
   PAMD_ARG_CHECK(bytes != nullptr, 4);
   const int s = gemm_splits(M, N, K);
-  *bytes = (s > 1) ? gemm_part_bytes(M, N, s) + gemm_cnt_bytes(M, N) : 0;
+  *bytes = (s > 1) ? (int64_t)s * M * N * (int64_t)sizeof(double) : 0;
   return PAMD_OK;
 }
 
@@ -424,24 +374,15 @@ extern "C" int pamd_gemm_f64(int transA, int transB, int64_t M, int64_t N, int64
   if (M == 0 || N == 0) return PAMD_OK;
   hipStream_t st = (hipStream_t)stream;
   int splits = gemm_splits(M, N, K);
-  if (splits > 1 && (workspace == nullptr || workspace_bytes < gemm_part_bytes(M, N, splits))) splits = 1;
+  if (splits > 1 && (workspace == nullptr || workspace_bytes < (int64_t)splits * M * N * (int64_t)sizeof(double)))
+    splits = 1;
   const int64_t kchunk = splits > 1 ? pamd_cdiv(pamd_cdiv(K, splits), GBK) * GBK : imax64(K, 1);
   if (splits > 1) splits = (int)pamd_cdiv(K, kchunk);
   double* part = splits > 1 ? (double*)workspace : nullptr;
-  // with room for the per-tile counters the last slice of each tile reduces
-  // in the kernel; with room for the partials only, a second kernel does
-  int* cnt = nullptr;
-  const int64_t tiles = pamd_cdiv(M, gemm_edge(M)) * pamd_cdiv(N, gemm_edge(N));
-  if (splits > 1 && workspace_bytes >= gemm_part_bytes(M, N, splits) + gemm_cnt_bytes(M, N)) {
-    cnt = reinterpret_cast<int*>((char*)workspace + gemm_part_bytes(M, N, splits));
-    zero_ints_kernel<<<(unsigned)pamd_cdiv(tiles, 256), 256, 0, st>>>(cnt, tiles);
-    PAMD_LAUNCH_CHECK();
-  }
   const int bm = gemm_edge(M), bn = gemm_edge(N);
   dim3 grid((unsigned)pamd_cdiv(N, bn), (unsigned)pamd_cdiv(M, bm), (unsigned)splits);
-#define PAMD_GEMM(TA, TB, BM, BN)                                                                              \
-  gemm_f64_kernel<TA, TB, BM, BN><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part, \
-                                                        cnt)
+#define PAMD_GEMM(TA, TB, BM, BN) \
+  gemm_f64_kernel<TA, TB, BM, BN><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part)
 #define PAMD_GEMM_T(BM, BN)                                  \
   do {                                                       \
     if (transA == 0 && transB == 0) PAMD_GEMM(0, 0, BM, BN); \
@@ -456,7 +397,7 @@ extern "C" int pamd_gemm_f64(int transA, int transB, int64_t M, int64_t N, int64
 #undef PAMD_GEMM_T
 #undef PAMD_GEMM
   PAMD_LAUNCH_CHECK();
-  if (splits > 1 && cnt == nullptr) {
+  if (splits > 1) {
     const int64_t tot = M * N;
     splitk_reduce_kernel<<<(unsigned)pamd_cdiv(tot, 256), 256, 0, st>>>(M, N, splits, alpha, part, beta, C, ldc);
     PAMD_LAUNCH_CHECK();

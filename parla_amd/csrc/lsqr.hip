@@ -46,9 +46,23 @@ struct LsqrConst {
 // scratch (<= 64 KiB of dynamic LDS)
 constexpr int64_t LSQR_LDS_MAX_N = 1900;
 
-constexpr int PASS_THREADS = 256;
-
-__host__ __device__ inline int lsqr_grid(int64_t m) { return (int)imax64(1, imin64(1024, pamd_cdiv(m, 64))); }
+// Pass workgroups of lsqr_waves(m, n) wavefronts, one wavefront per row at a
+// time: four, or more where that still leaves at least one workgroup per CU
+// (wider workgroups leave fewer partials for the tail to sum; the register
+// tile of 4 * NPL doubles per lane bounds the width).
+__host__ __device__ inline int lsqr_npl(int64_t n) {
+  return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : n <= 1024 ? 16 : 0;
+}
+// wavefronts over the rows of A: one per 16 rows, at most 4096 (16 per CU)
+__host__ __device__ inline int64_t lsqr_total_waves(int64_t m) { return imax64(1, imin64(4096, pamd_cdiv(m, 16))); }
+__host__ __device__ inline int lsqr_waves(int64_t m, int64_t n) {
+  const int npl = lsqr_npl(n);
+  const int wide = (npl >= 1 && npl <= 8) ? 16 : (npl == 16 ? 8 : 4);
+  return pamd_cdiv(lsqr_total_waves(m), wide) >= 256 ? wide : 4;
+}
+__host__ __device__ inline int lsqr_grid(int64_t m, int64_t n) {
+  return (int)pamd_cdiv(lsqr_total_waves(m), lsqr_waves(m, n));
+}
 
 struct LsqrLayout {
   int64_t st, u, v, w, work, y, pv, pb, xs, total;
@@ -56,7 +70,7 @@ struct LsqrLayout {
 
 __host__ __device__ inline LsqrLayout lsqr_layout(int64_t m, int64_t n) {
   LsqrLayout L;
-  const int G = lsqr_grid(m);
+  const int G = lsqr_grid(m, n);
   L.st = 0;
   L.u = 64;
   L.v = L.u + m + n;
@@ -74,10 +88,14 @@ __host__ __device__ inline LsqrLayout lsqr_layout(int64_t m, int64_t n) {
 // pass over A
 // ---------------------------------------------------------------------------
 // NPL = columns per lane held in registers (n <= 64 * NPL); NPL = 0: any n,
-// the row is read twice (the second read hits cache).
-template <int NPL>
-__global__ __launch_bounds__(PASS_THREADS) void lsqr_pass_kernel(LsqrConst K, const double* __restrict__ A,
-                                                                 double* __restrict__ ws) {
+// the row is read twice (the second read hits cache).  WAVES wavefronts per
+// workgroup (lsqr_waves); the workgroup's partial of A^T u' is the sum of its
+// wavefronts' in wavefront order.
+template <int NPL, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void lsqr_pass_kernel(LsqrConst K, const double* __restrict__ A,
+                                                               double* __restrict__ ws) {
+  constexpr int THREADS = 64 * WAVES;
+  constexpr int RED_ROWS = 4;  // wavefronts whose register tiles share LDS at a time
   LsqrState* S = reinterpret_cast<LsqrState*>(ws);
   if (S->done) return;
   const LsqrLayout L = lsqr_layout(K.m, K.n);
@@ -89,11 +107,11 @@ __global__ __launch_bounds__(PASS_THREADS) void lsqr_pass_kernel(LsqrConst K, co
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t n = K.n;
-  const int64_t gw = (int64_t)gridDim.x * (PASS_THREADS / 64);
+  const int64_t gw = (int64_t)gridDim.x * WAVES;
   double ssq_new = 0.0, ssq_in = 0.0;
 
-  __shared__ double red[PASS_THREADS / 64][64 * (NPL > 0 ? NPL : 1)];
-  __shared__ double sred[PASS_THREADS / 64][2];
+  __shared__ double red[RED_ROWS][64 * (NPL > 0 ? NPL : 1)];
+  __shared__ double sred[WAVES][2];
 
   if constexpr (NPL > 0) {
     double wr[NPL], acc[NPL];
@@ -103,7 +121,7 @@ __global__ __launch_bounds__(PASS_THREADS) void lsqr_pass_kernel(LsqrConst K, co
       wr[q] = (c < n) ? w[c] : 0.0;
       acc[q] = 0.0;
     }
-    int64_t row = (int64_t)blockIdx.x * (PASS_THREADS / 64) + wave;
+    int64_t row = (int64_t)blockIdx.x * WAVES + wave;
     double a[NPL];
     if (row < K.m) {
 #pragma unroll
@@ -139,24 +157,44 @@ __global__ __launch_bounds__(PASS_THREADS) void lsqr_pass_kernel(LsqrConst K, co
         for (int q = 0; q < NPL; ++q) a[q] = an[q];
       }
     }
-#pragma unroll
-    for (int q = 0; q < NPL; ++q) red[wave][lane + 64 * q] = acc[q];
     if (lane == 0) {
       sred[wave][0] = ssq_new;
       sred[wave][1] = ssq_in;
     }
-    __syncthreads();
-    for (int64_t c = threadIdx.x; c < n; c += PASS_THREADS) {
-      double t = 0.0;
+    // the wavefronts' tiles pass through LDS RED_ROWS at a time; thread
+    // columns c = threadIdx.x + THREADS * j add them in wavefront order
+    constexpr int CPT = (64 * NPL + THREADS - 1) / THREADS;
+    double t[CPT];
 #pragma unroll
-      for (int wv = 0; wv < PASS_THREADS / 64; ++wv) t += red[wv][c];
-      pv[(int64_t)blockIdx.x * n + c] = t;
+    for (int j = 0; j < CPT; ++j) t[j] = 0.0;
+#pragma unroll
+    for (int w0 = 0; w0 < WAVES; w0 += RED_ROWS) {
+      if (wave >= w0 && wave < w0 + RED_ROWS) {
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) red[wave - w0][lane + 64 * q] = acc[q];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        const int c = threadIdx.x + THREADS * j;
+        if (c < 64 * NPL) {
+#pragma unroll
+          for (int wv = 0; wv < RED_ROWS; ++wv)
+            if (w0 + wv < WAVES) t[j] += red[wv][c];
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const int c = threadIdx.x + THREADS * j;
+      if (c < n) pv[(int64_t)blockIdx.x * n + c] = t[j];
     }
   } else {
-    for (int64_t c = threadIdx.x; c < n; c += PASS_THREADS) pv[(int64_t)blockIdx.x * n + c] = 0.0;
+    for (int64_t c = threadIdx.x; c < n; c += THREADS) pv[(int64_t)blockIdx.x * n + c] = 0.0;
     __syncthreads();
     // uniform trip count across the block (the loop body has barriers)
-    for (int64_t base = (int64_t)blockIdx.x * (PASS_THREADS / 64); base < K.m; base += gw) {
+    for (int64_t base = (int64_t)blockIdx.x * WAVES; base < K.m; base += gw) {
       const int64_t row = base + wave;
       const bool valid = row < K.m;
       const double* ar = A + (valid ? row : 0) * K.lda;
@@ -172,7 +210,7 @@ __global__ __launch_bounds__(PASS_THREADS) void lsqr_pass_kernel(LsqrConst K, co
         if (lane == 0) u[row] = un;
       }
       // accumulate into this block's partial, one wave at a time (fixed order)
-      for (int wv = 0; wv < PASS_THREADS / 64; ++wv) {
+      for (int wv = 0; wv < WAVES; ++wv) {
         __syncthreads();
         if (wv == wave && valid)
           for (int64_t c = lane; c < n; c += 64) pv[(int64_t)blockIdx.x * n + c] += un * ar[c];
@@ -186,7 +224,7 @@ __global__ __launch_bounds__(PASS_THREADS) void lsqr_pass_kernel(LsqrConst K, co
   }
   if (threadIdx.x == 0) {
     double a0 = 0.0, a1 = 0.0;
-    for (int wv = 0; wv < PASS_THREADS / 64; ++wv) {
+    for (int wv = 0; wv < WAVES; ++wv) {
       a0 += sred[wv][0];
       a1 += sred[wv][1];
     }
@@ -220,38 +258,48 @@ __device__ inline void sym_ortho(double a, double b, double& c, double& s, doubl
   }
 }
 
-// y = sum of pass partials (+ lifted rows) and the ssq totals.  Columns go
-// in chunks of 64 (lanes), the partial rows g are split over the 16 waves and
-// combined in wave order; every thread gets the totals, y is complete on return.
+// y = sum of pass partials (+ lifted rows) and the ssq totals.  The columns
+// go in chunks of 64 (lanes); wpc wavefronts share a chunk, wavefront `sub`
+// of it summing partials g = sub, sub + wpc, ... in order with sixteen loads
+// in flight, and the wpc sums are added in order.  Every thread gets the
+// totals; y is complete on return.
 __device__ inline void gather_partials(const LsqrConst& K, double* ws, const LsqrLayout& L, const LsqrState* S,
                                        double* red, double (*cred)[64], double& ssq_new, double& ssq_in) {
+  constexpr int GL = 16;
   const int64_t n = K.n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const double* pv = ws + L.pv;
+  const int ng = K.grid;
   double* y = ws + L.y;
   double a = 0.0, b = 0.0;
-  for (int g = threadIdx.x; g < K.grid; g += blockDim.x) {
+  for (int g = threadIdx.x; g < ng; g += blockDim.x) {
     a += ws[L.pb + 2 * g];
     b += ws[L.pb + 2 * g + 1];
   }
   const double c1 = S->c1, c2 = S->c2;
-  for (int64_t c0 = 0; c0 < n; c0 += 64) {
-    const int64_t c = c0 + lane;
+  const int nch = (int)pamd_cdiv(n, 64);
+  const int wpc = max(1, nw / nch);  // wavefronts per column chunk
+  const int cpp = nw / wpc;          // chunks per pass
+  const int sub = wave % wpc;
+  for (int ch0 = 0; ch0 < nch; ch0 += cpp) {
+    const int ch = ch0 + wave / wpc;
+    const int64_t c = (int64_t)ch * 64 + lane;
+    const bool mine = (wave / wpc < cpp) && ch < nch && c < n;
     double t = 0.0;
-    if (c < n)  // partials g = wave, wave + nw, ... in order, eight loads in flight
-      for (int g = wave; g < K.grid; g += 8 * nw) {
-        double pg[8];
+    if (mine)
+      for (int g = sub; g < ng; g += GL * wpc) {
+        double pg[GL];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) pg[u] = (g + u * nw < K.grid) ? pv[(int64_t)(g + u * nw) * n + c] : 0.0;
+        for (int u = 0; u < GL; ++u) pg[u] = (g + u * wpc < ng) ? pv[(int64_t)(g + u * wpc) * n + c] : 0.0;
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-          if (g + u * nw < K.grid) t += pg[u];
+        for (int u = 0; u < GL; ++u)
+          if (g + u * wpc < ng) t += pg[u];
       }
     cred[wave][lane] = t;
     __syncthreads();
-    if (wave == 0 && c < n) {
+    if (mine && sub == 0) {
       double tot = 0.0;
-      for (int wv = 0; wv < nw; ++wv) tot += cred[wv][lane];
+      for (int s2 = 0; s2 < wpc; ++s2) tot += cred[wave + s2][lane];
       if (K.sd > 0.0) {
         double* ul = ws + L.u + K.m;
         const double uin = ul[c];
@@ -518,19 +566,22 @@ __global__ __launch_bounds__(1024) void lsqr_xorig_kernel(LsqrConst K, const dou
 }
 
 int launch_pass(const LsqrConst& K, const double* A, double* ws, hipStream_t st) {
-  const int64_t n = K.n;
-  if (n <= 64)
-    lsqr_pass_kernel<1><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
-  else if (n <= 128)
-    lsqr_pass_kernel<2><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
-  else if (n <= 256)
-    lsqr_pass_kernel<4><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
-  else if (n <= 512)
-    lsqr_pass_kernel<8><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
-  else if (n <= 1024)
-    lsqr_pass_kernel<16><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
+  const int npl = lsqr_npl(K.n);
+  const bool wide = lsqr_waves(K.m, K.n) > 4;
+#define PAMD_PASS(NPL, WAVES) lsqr_pass_kernel<NPL, WAVES><<<K.grid, 64 * WAVES, 0, st>>>(K, A, ws)
+  if (npl == 1)
+    wide ? PAMD_PASS(1, 16) : PAMD_PASS(1, 4);
+  else if (npl == 2)
+    wide ? PAMD_PASS(2, 16) : PAMD_PASS(2, 4);
+  else if (npl == 4)
+    wide ? PAMD_PASS(4, 16) : PAMD_PASS(4, 4);
+  else if (npl == 8)
+    wide ? PAMD_PASS(8, 16) : PAMD_PASS(8, 4);
+  else if (npl == 16)
+    wide ? PAMD_PASS(16, 8) : PAMD_PASS(16, 4);
   else
-    lsqr_pass_kernel<0><<<K.grid, PASS_THREADS, 0, st>>>(K, A, ws);
+    PAMD_PASS(0, 4);
+#undef PAMD_PASS
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }
@@ -546,8 +597,13 @@ extern "C" int pamd_lsqr_workspace(int64_t m, int64_t n, int64_t iter_lim, int64
   return PAMD_OK;
 }
 
-extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, double* x_orig, double* var,
-                                 double* arnorms, pamd_lsqr_result* result, void* workspace, void* stream) {
+namespace {
+
+// Checks the arguments shared by the blocking and the asynchronous entry
+// point, fills K and the dynamic LDS size, and launches the set-up (init,
+// first pass over A, tail phase 0).
+int lsqr_start(const pamd_lsqr_problem* prob, double* x, double* var, double* arnorms, const void* result,
+               void* workspace, hipStream_t st, LsqrConst& K, size_t& shmem) {
   PAMD_ARG_CHECK(prob != nullptr, 1);
   const pamd_lsqr_problem& P = *prob;
   PAMD_ARG_CHECK(P.m >= 0 && P.n >= 1, 1);
@@ -561,8 +617,6 @@ extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, doubl
   PAMD_ARG_CHECK(arnorms != nullptr || P.iter_lim <= 0, 5);
   PAMD_ARG_CHECK(result != nullptr, 6);
   PAMD_ARG_CHECK(workspace != nullptr, 7);
-  hipStream_t st = (hipStream_t)stream;
-  LsqrConst K;
   K.m = P.m;
   K.n = P.n;
   K.lda = P.lda;
@@ -575,19 +629,46 @@ extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, doubl
   K.has_r = P.R != nullptr;
   K.calc_var = P.calc_var != 0;
   K.has_x0 = P.x0 != nullptr;
-  K.grid = lsqr_grid(P.m);
+  K.grid = lsqr_grid(P.m, P.n);
   K.ldr = P.ldr;
   K.xs_global = (K.has_r && P.n > LSQR_LDS_MAX_N) ? 1 : 0;
   double* ws = (double*)workspace;
-  const size_t shmem =
-      (size_t)(K.has_r ? ((K.xs_global ? 0 : P.n) + TRSV_SCRATCH_DOUBLES) : 1) * sizeof(double);
-
+  shmem = (size_t)(K.has_r ? ((K.xs_global ? 0 : P.n) + TRSV_SCRATCH_DOUBLES) : 1) * sizeof(double);
   lsqr_init_kernel<<<1, 1024, shmem, st>>>(K, P.R, P.b, P.x0, x, var, ws);
   PAMD_LAUNCH_CHECK();
   int rc = launch_pass(K, P.A, ws, st);
   if (rc) return rc;
   lsqr_tail_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, var, arnorms, ws);  // phase 0: set-up
   PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+// the result fields from the device state (one thread)
+__global__ void lsqr_result_kernel(const double* __restrict__ ws, pamd_lsqr_result* __restrict__ r) {
+  if (threadIdx.x != 0) return;
+  const LsqrState* S = reinterpret_cast<const LsqrState*>(ws);
+  r->istop = S->istop;
+  r->itn = S->itn;
+  r->r1norm = S->r1norm;
+  r->r2norm = S->r2norm;
+  r->anorm = S->anorm;
+  r->acond = S->acond;
+  r->arnorm = S->arnorm;
+  r->xnorm = S->xnorm;
+  r->n_log = S->early ? -1 : S->itn;
+}
+
+}  // namespace
+
+extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, double* x_orig, double* var,
+                                 double* arnorms, pamd_lsqr_result* result, void* workspace, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  LsqrConst K;
+  size_t shmem = 0;
+  int rc = lsqr_start(prob, x, var, arnorms, result, workspace, st, K, shmem);
+  if (rc) return rc;
+  const pamd_lsqr_problem& P = *prob;
+  double* ws = (double*)workspace;
 
   LsqrState hs;
   int64_t batch = 4;
@@ -619,5 +700,30 @@ extern "C" int pamd_lsqr_run_f64(const pamd_lsqr_problem* prob, double* x, doubl
   result->arnorm = hs.arnorm;
   result->xnorm = hs.xnorm;
   result->n_log = hs.early ? -1 : hs.itn;
+  return PAMD_OK;
+}
+
+extern "C" int pamd_lsqr_launch_f64(const pamd_lsqr_problem* prob, double* x, double* x_orig, double* var,
+                                    double* arnorms, pamd_lsqr_result* result, void* workspace, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  LsqrConst K;
+  size_t shmem = 0;
+  int rc = lsqr_start(prob, x, var, arnorms, result, workspace, st, K, shmem);
+  if (rc) return rc;
+  const pamd_lsqr_problem& P = *prob;
+  double* ws = (double*)workspace;
+  // every iteration is launched; those after the stop return at once
+  for (int64_t it = 0; it < P.iter_lim; ++it) {
+    rc = launch_pass(K, P.A, ws, st);
+    if (rc) return rc;
+    lsqr_tail_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, var, arnorms, ws);
+    PAMD_LAUNCH_CHECK();
+  }
+  if (x_orig) {
+    lsqr_xorig_kernel<<<1, 1024, shmem, st>>>(K, P.R, x, x_orig, ws);
+    PAMD_LAUNCH_CHECK();
+  }
+  lsqr_result_kernel<<<1, 64, 0, st>>>(ws, result);
+  PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

@@ -16,11 +16,28 @@
 #define PAMD_TRSV_MAX_N 2048
 constexpr int TRSV_SCRATCH_DOUBLES = 64 * 65 + 16 * 64 + 64;
 
+// The 64 x 64 diagonal block at rows / columns r0.. of R (bs = its size): a
+// thread's share held in registers, element e = threadIdx.x + blockDim.x * k
+// for k < DPT (row e / 64, column e % 64); zero outside the block.
+template <int DPT>
+__device__ __forceinline__ void trsv_load_diag(const double* __restrict__ R, int64_t ldr, int r0, int bs,
+                                               double (&d)[DPT]) {
+#pragma unroll
+  for (int k = 0; k < DPT; ++k) {
+    const int e = threadIdx.x + blockDim.x * k;
+    const int i = e >> 6, j = e & 63;
+    d[k] = (e < 64 * 64 && i < bs && j < bs) ? R[(int64_t)(r0 + i) * ldr + r0 + j] : 0.0;
+  }
+}
+
 // All threads of the block (blockDim.x a multiple of 64, at most 1024) must
 // call this.  trans = 0 solves R x = b, trans = 1 solves R^T x = b; xs holds b
 // on entry and x on exit, in LDS or (for n beyond what LDS holds) in global
 // memory that only this workgroup touches.  scratch: TRSV_SCRATCH_DOUBLES
-// doubles of LDS.
+// doubles of LDS.  DPT: diagonal-block elements a thread keeps in registers;
+// 4096 / blockDim.x covers the whole block, a smaller value is still correct
+// (the rest is read from R when it is stored to LDS).
+template <int DPT = 4>
 __device__ inline void trsv_upper_block(int trans, int n, const double* __restrict__ R, int64_t ldr,
                                         double* xs, double* scratch) {
   double(*Rd)[65] = reinterpret_cast<double(*)[65]>(scratch);
@@ -31,18 +48,22 @@ __device__ inline void trsv_upper_block(int trans, int n, const double* __restri
   const int wave = tid >> 6;
   const int nwaves = blockDim.x >> 6;
   const int nb = (n + 63) / 64;
+  // The diagonal block of the next step is loaded into registers while the
+  // current one is solved, and stored to LDS after the next step's
+  // off-diagonal product: its memory latency is hidden twice over.
+  double dnext[DPT];
+  {
+    const int blk = trans ? 0 : (nb - 1);
+    trsv_load_diag(R, ldr, blk * 64, min(64, n - blk * 64), dnext);
+  }
   for (int bi = 0; bi < nb; ++bi) {
     const int blk = trans ? bi : (nb - 1 - bi);
     const int r0 = blk * 64;
     const int bs = min(64, n - r0);
     const int r1 = r0 + bs;
-#pragma unroll 4
-    for (int e = tid; e < 64 * 64; e += blockDim.x) {
-      const int i = e >> 6, j = e & 63;
-      const double v = (i < bs && j < bs) ? R[(int64_t)(r0 + i) * ldr + r0 + j] : 0.0;
-      Rd[i][j] = v;
-      if (i == j) rdiag[i] = (i < bs) ? 1.0 / v : 0.0;
-    }
+    double dcur[DPT];
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) dcur[k] = dnext[k];
     if (!trans) {
       // xs[r0:r1] -= R[r0:r1, r1:n] @ xs[r1:n]; a lane's terms in order, the
       // row values of four of them loaded before they are used
@@ -82,7 +103,26 @@ __device__ inline void trsv_upper_block(int trans, int n, const double* __restri
         }
       red[wave][lane] = s;
     }
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+      const int e = tid + blockDim.x * k;
+      if (e < 64 * 64) {
+        const int i = e >> 6, j = e & 63;
+        Rd[i][j] = (i < bs && j < bs) ? dcur[k] : ((i == j) ? 1.0 : 0.0);
+        if (i == j) rdiag[i] = (i < bs) ? 1.0 / dcur[k] : 0.0;
+      }
+    }
+    for (int e = tid + blockDim.x * DPT; e < 64 * 64; e += blockDim.x) {
+      const int i = e >> 6, j = e & 63;
+      const double v = (i < bs && j < bs) ? R[(int64_t)(r0 + i) * ldr + r0 + j] : 0.0;
+      Rd[i][j] = (i < bs && j < bs) ? v : ((i == j) ? 1.0 : 0.0);
+      if (i == j) rdiag[i] = (i < bs) ? 1.0 / v : 0.0;
+    }
     __syncthreads();
+    if (bi + 1 < nb) {
+      const int nblk = trans ? bi + 1 : (nb - 2 - bi);
+      trsv_load_diag(R, ldr, nblk * 64, min(64, n - nblk * 64), dnext);
+    }
     if (wave == 0) {
       double y = (lane < bs) ? xs[r0 + lane] : 0.0;
       const double rd = rdiag[lane];  // lane i holds 1 / R_ii

@@ -893,19 +893,10 @@ def sjlt_right(rowptr, cols, vals, k, A, out=None):
 # LSQR
 # ---------------------------------------------------------------------------
 
-def lsqr_run(A, b, R=None, sqrt_delta=0.0, x0=None, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8,
-             iter_lim=None, calc_var=False, want_orig=False):
-    """Device LSQR on [A; sqrt_delta I] R^{-1} with rhs [b; 0].
-
-    Returns (x, var, arnorms, result) -- or (x, x_orig, var, arnorms, result)
-    with ``want_orig``, x_orig = R^{-1} x computed by the iteration's own
-    triangular substitution."""
+def _lsqr_problem(A, b, R, sqrt_delta, x0, damp, atol, btol, conlim, iter_lim, calc_var):
+    """The pamd_lsqr_problem struct and the tensors it points to (kept alive by the caller)."""
     A = A if A.stride(-1) == 1 else A.contiguous()
     m, n = A.shape
-    dev = A.device
-    if iter_lim is None:
-        iter_lim = 2 * n
-    iter_lim = int(iter_lim)
     b = b.contiguous()
     prob = _lib.LsqrProblem()
     prob.m, prob.n = m, n
@@ -926,6 +917,20 @@ def lsqr_run(A, b, R=None, sqrt_delta=0.0, x0=None, damp=0.0, atol=1e-8, btol=1e
     prob.damp, prob.atol, prob.btol, prob.conlim = float(damp), float(atol), float(btol), float(conlim)
     prob.iter_lim = iter_lim
     prob.calc_var = int(bool(calc_var))
+    return prob, (A, b, R, x0)
+
+
+def lsqr_run(A, b, R=None, sqrt_delta=0.0, x0=None, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8,
+             iter_lim=None, calc_var=False, want_orig=False):
+    """Device LSQR on [A; sqrt_delta I] R^{-1} with rhs [b; 0].
+
+    Returns (x, var, arnorms, result) -- or (x, x_orig, var, arnorms, result)
+    with ``want_orig``, x_orig = R^{-1} x computed by the iteration's own
+    triangular substitution."""
+    m, n = A.shape
+    dev = A.device
+    iter_lim = int(2 * n if iter_lim is None else iter_lim)
+    prob, keep = _lsqr_problem(A, b, R, sqrt_delta, x0, damp, atol, btol, conlim, iter_lim, calc_var)
     x = torch.empty(n, dtype=F64, device=dev)
     xo = torch.empty(n, dtype=F64, device=dev) if want_orig else None
     var = torch.zeros(n, dtype=F64, device=dev)
@@ -934,6 +939,48 @@ def lsqr_run(A, b, R=None, sqrt_delta=0.0, x0=None, damp=0.0, atol=1e-8, btol=1e
     res = _lib.LsqrResult()
     check(lib().pamd_lsqr_run_f64(ctypes.byref(prob), _p(x), _p(xo), _p(var), _p(arn), ctypes.byref(res),
                                   _p(ws), stream(dev)), "pamd_lsqr_run_f64")
+    del keep
     if want_orig:
         return x, xo, var, arn, res
     return x, var, arn, res
+
+
+LSQR_RESULT_BYTES = ctypes.sizeof(_lib.LsqrResult)
+
+
+def lsqr_result_from_device(t):
+    """A pamd_lsqr_result read back from the device bytes pamd_lsqr_launch_f64 wrote."""
+    raw = bytes(t.cpu().numpy().tobytes())
+    return _lib.LsqrResult.from_buffer_copy(raw)
+
+
+def lsqr_launch(A, b, R=None, sqrt_delta=0.0, x0=None, damp=0.0, atol=1e-8, btol=1e-8, conlim=1e8,
+                iter_lim=None, calc_var=False, out=None):
+    """LSQR as launches only (graph-capturable): every iteration up to
+    iter_lim is launched and the result stays on the device.
+
+    ``out`` = (x, x_orig or None, var, arnorms, result bytes, workspace) from a
+    previous call (or from :func:`lsqr_launch_buffers`) is reused, so a
+    captured graph owns no allocation.  Returns that tuple."""
+    m, n = A.shape
+    dev = A.device
+    iter_lim = int(2 * n if iter_lim is None else iter_lim)
+    if out is None:
+        out = lsqr_launch_buffers(m, n, iter_lim, dev)
+    x, xo, var, arn, resb, ws = out
+    prob, keep = _lsqr_problem(A, b, R, sqrt_delta, x0, damp, atol, btol, conlim, iter_lim, calc_var)
+    check(lib().pamd_lsqr_launch_f64(ctypes.byref(prob), _p(x), _p(xo), _p(var), _p(arn), _p(resb), _p(ws),
+                                     stream(dev)), "pamd_lsqr_launch_f64")
+    del keep
+    return out
+
+
+def lsqr_launch_buffers(m, n, iter_lim, dev, want_orig=False):
+    """Outputs and workspace of :func:`lsqr_launch` for an m x n problem."""
+    x = torch.empty(n, dtype=F64, device=dev)
+    xo = torch.empty(n, dtype=F64, device=dev) if want_orig else None
+    var = torch.zeros(n, dtype=F64, device=dev)
+    arn = torch.zeros(max(iter_lim, 1), dtype=F64, device=dev)  # entries past the stop stay 0
+    resb = torch.zeros(LSQR_RESULT_BYTES, dtype=torch.uint8, device=dev)
+    ws = workspace(_query(lib().pamd_lsqr_workspace, m, n, iter_lim), dev)
+    return x, xo, var, arn, resb, ws

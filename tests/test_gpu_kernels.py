@@ -76,44 +76,6 @@ def test_gemm_alpha_beta_and_strided_output(ops):
     assert np.allclose(host(out), ref, rtol=1e-13, atol=1e-13)
 
 
-@pytest.mark.parametrize("ta,M,N,K", [(0, 8192, 266, 4096), (1, 4096, 266, 8192), (0, 300, 100, 9000),
-                                      (1, 97, 266, 3000), (0, 266, 266, 266)])
-def test_gemm_fused_splitk_reduction_bitwise(ops, ta, M, N, K):
-    """Split K with the in-kernel reduction (workspace with room for the
-    per-tile counters) gives the same bits as the separate reduction kernel
-    (workspace for the partials only), on two calls in a row and with beta."""
-    import ctypes
-    _say(f"gemm split-K fused vs separate ta={ta} M={M} N={N} K={K}")
-    lib = ops.lib()
-    rng = np.random.default_rng(M + N + K)
-    A = dev(rng.standard_normal((K, M) if ta else (M, K)))
-    B = dev(rng.standard_normal((K, N)))
-    C0 = dev(rng.standard_normal((M, N)))
-    full = ops._query(lib.pamd_gemm_workspace, M, N, K)
-    if full == 0:
-        pytest.skip("no split for this shape")
-    def edge(d):   # the kernel's tile edge: 96 where it pads at least 5% less than 64
-        p64, p96 = -(-d // 64) * 64, -(-d // 96) * 96
-        return 96 if p96 * 20 <= p64 * 19 else 64
-
-    tiles = -(-M // edge(M)) * -(-N // edge(N))
-    part_only = full - 4 * tiles     # exactly the partials: no room for the counters
-    assert part_only > 0 and part_only % (8 * M * N) == 0
-    st = ops.stream(A.device)
-    outs = []
-    for wsb in (full, part_only, full):
-        ws = ops.workspace(wsb, A.device)
-        C = C0.clone()
-        rc = lib.pamd_gemm_f64(ta, 0, M, N, K, 1.5, ops._p(A), A.stride(0), ops._p(B), B.stride(0), -0.5,
-                               ops._p(C), C.stride(0), ops._p(ws), wsb, st)
-        assert rc == 0
-        outs.append(host(C))
-    np.testing.assert_array_equal(outs[0], outs[1])
-    np.testing.assert_array_equal(outs[2], outs[1])
-    ref = 1.5 * ((host(A).T if ta else host(A)) @ host(B)) - 0.5 * host(C0)
-    assert rel(outs[0], ref) < 50 * EPS * np.sqrt(K)
-
-
 def test_mfma_f64_lane_layout(ops):
     # a 16 x 16 x 4 product exercises exactly one MFMA per wave: any layout
     # mistake permutes rows/columns and fails the bit-for-bit comparison of
