@@ -282,7 +282,7 @@ __global__ __launch_bounds__(256) void bt_trsv_kernel(const Desc* __restrict__ D
   double* xs = smem;
   for (int i = threadIdx.x; i < n; i += blockDim.x) xs[i] = v[d.nv_off + i];
   __syncthreads();
-  trsv_upper_block<16>(trans, n, Rp + d.r_off, n, xs, smem + BT_MAXN);
+  trsv_upper_block<8>(trans, n, Rp + d.r_off, n, xs, smem + BT_MAXN);
   for (int i = threadIdx.x; i < n; i += blockDim.x) v[d.nv_off + i] = xs[i];
 }
 

@@ -445,7 +445,7 @@ __global__ __launch_bounds__(256) void trtri_kernel(int64_t n, const double* __r
   double* xs = smem;
   for (int i = threadIdx.x; i < sz; i += blockDim.x) xs[i] = (i == sz - 1) ? 1.0 : 0.0;
   __syncthreads();
-  trsv_upper_block<16>(0, sz, R, ldr, xs, smem + sz);
+  trsv_upper_block<8>(0, sz, R, ldr, xs, smem + sz);
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Ri[i * ldi + j] = (i <= j) ? xs[i] : 0.0;
 }
 
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(256) void trtri_batched_kernel(const int64_t* __res
   double* xs = smem;
   for (int i = threadIdx.x; i < sz; i += blockDim.x) xs[i] = (i == sz - 1) ? 1.0 : 0.0;
   __syncthreads();
-  trsv_upper_block<16>(0, sz, R, n, xs, smem + sz);
+  trsv_upper_block<8>(0, sz, R, n, xs, smem + sz);
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Ri[i * n + j] = (i <= j) ? xs[i] : 0.0;
 }
 
