@@ -34,6 +34,11 @@ Parity status of each function (how it is pinned, see tests/test_oracle_*.py):
   leverage_scores -- sum = n, invariance under right scaling, closed forms.
 * bf16_pieces, split_product -- the reconstruction bound of the pieces, and
   exact rational evaluation of the piece products on small cases.
+* philox4x32_10 -- the known-answer vectors published with Random123 (Salmon
+  et al., SC'11); u53_open0 -- its range and exactness at the end points;
+  device_gaussian_fill, device_sjlt_draw, device_sample_rows (the device
+  draws, which have no counterpart in the reference) -- moments,
+  distinctness, offset invariance and the frequencies of the inverse CDF.
 """
 import time
 import warnings
@@ -55,7 +60,8 @@ __all__ = [
     "MLP_NPARAM", "bf16_pieces", "bf16_values", "split_product",
     "svd_right_precond", "pcss2_under", "spu1", "sso1", "pcg", "pcss1", "pcss2_saddle", "sps1", "sps2",
     "project_out", "qb2", "qb3", "evd2", "sparse_sign_operator", "generate_srct", "apply_srct",
-    "srct_matrix",
+    "srct_matrix", "philox4x32_10", "u53_open0", "device_gaussian_fill", "device_sjlt_draw",
+    "device_sample_rows",
 ]
 
 
@@ -1287,3 +1293,133 @@ def evd2(A, k, tol, over, rng, sk_op):
             comp.append(i)
     r = min(comp)
     return V[:, :r], (sigma ** 2)[:r] - nu
+
+
+# --------------------------------------------------------------------------
+# Device draws: counter-based streams of the GPU library (no counterpart in
+# the reference, which draws with numpy's Generator).  Restated so that the
+# device's Gaussian sketch, SJLT draws and weighted row sampling are checked
+# value for value.
+# --------------------------------------------------------------------------
+
+_M32 = np.uint64(0xFFFFFFFF)
+
+
+def philox4x32_10(c0, c1, c2, c3, k0, k1):
+    """Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
+    as easy as 1, 2, 3", SC'11): ten rounds of two 32 x 32 -> 64-bit
+    multiplications (0xD2511F53, 0xCD9E8D57) with the Weyl key schedule
+    (0x9E3779B9, 0xBB67AE85).  Counter words and keys broadcast; returns four
+    uint32 arrays."""
+    c = [np.asarray(x, dtype=np.uint64) & _M32 for x in (c0, c1, c2, c3)]
+    c0, c1, c2, c3 = np.broadcast_arrays(*c)
+    k0 = np.asarray(k0, dtype=np.uint64) & _M32
+    k1 = np.asarray(k1, dtype=np.uint64) & _M32
+    for _ in range(10):
+        p0 = np.uint64(0xD2511F53) * c0
+        p1 = np.uint64(0xCD9E8D57) * c2
+        c0, c1, c2, c3 = (p1 >> np.uint64(32)) ^ c1 ^ k0, p1 & _M32, (p0 >> np.uint64(32)) ^ c3 ^ k1, p0 & _M32
+        k0 = (k0 + np.uint64(0x9E3779B9)) & _M32
+        k1 = (k1 + np.uint64(0xBB67AE85)) & _M32
+    return tuple(x.astype(np.uint32) for x in (c0, c1, c2, c3))
+
+
+def u53_open0(hi, lo):
+    """The uniform in (0, 1] of two 32-bit words: (top 53 bits + 1) * 2^-53."""
+    b = ((np.asarray(hi, dtype=np.uint64) << np.uint64(32)) | np.asarray(lo, dtype=np.uint64)) >> np.uint64(11)
+    return (b.astype(np.float64) + 1.0) * 2.0 ** -53
+
+
+def _seed_words(seed):
+    seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    return seed & 0xFFFFFFFF, seed >> 32
+
+
+def device_gaussian_fill(rows, cols, scale, seed, offset=0):
+    """The device Gaussian sketch (pamd_gaussian_fill_f64): entry (i, j) is
+    draw e = offset + i * cols + j; draws 2p and 2p + 1 are the Box-Muller pair
+    of Philox(counter (p, p >> 32, 0x6A09E667, 0), key seed):
+    r = sqrt(-2 ln u1), z = (r cos 2 pi u2, r sin 2 pi u2)."""
+    k0, k1 = _seed_words(seed)
+    e = np.uint64(offset) + np.arange(rows * cols, dtype=np.uint64)
+    pair = e >> np.uint64(1)
+    x, y, z, w = philox4x32_10(pair & _M32, pair >> np.uint64(32), 0x6A09E667, 0, k0, k1)
+    u1, u2 = u53_open0(x, y), u53_open0(z, w)
+    rad = np.sqrt(-2.0 * np.log(u1))
+    odd = (e & np.uint64(1)).astype(bool)
+    val = np.where(odd, rad * np.sin(2.0 * np.pi * u2), rad * np.cos(2.0 * np.pi * u2))
+    return (scale * val).reshape(rows, cols)
+
+
+def device_sjlt_draw(n_short, n_long, k, seed):
+    """The device SJLT draw (pamd_sjlt_draw): column j's entry l is row
+    floor(x * n_short / 2^32) and sign bit y & 1 of Philox(counter (j,
+    j >> 32, l, 0x510E527F + attempt), key seed), redrawn (attempt + 1) while
+    the row repeats an earlier one of the column (with replacement when
+    n_short < k; after 64 repeats the first unused row).  Returns rows (n_long
+    x k, int32) and values +-1/sqrt(k) (negative where the sign bit is set)."""
+    k0, k1 = _seed_words(seed)
+    rows = np.zeros((n_long, k), dtype=np.int32)
+    neg = np.zeros((n_long, k), dtype=bool)
+    with_repl = n_short < k
+    j = np.arange(n_long, dtype=np.uint64)
+    for l in range(k):
+        todo = np.ones(n_long, dtype=bool)
+        for attempt in range(64):
+            x, y, _, _ = philox4x32_10(j & _M32, j >> np.uint64(32), l, 0x510E527F + attempt, k0, k1)
+            r = ((x.astype(np.uint64) * np.uint64(n_short)) >> np.uint64(32)).astype(np.int32)
+            dup = np.zeros(n_long, dtype=bool) if with_repl else np.any(rows[:, :l] == r[:, None], axis=1)
+            take = todo & ~dup
+            rows[take, l] = r[take]
+            neg[take, l] = (y[take] & 1).astype(bool)
+            todo &= dup
+            if not todo.any():
+                break
+        for jj in np.nonzero(todo)[0]:  # 64 straight repeats: first unused row
+            used = set(rows[jj, :l].tolist())
+            rows[jj, l] = next(r for r in range(n_short) if r not in used)
+            neg[jj, l] = bool(y[jj] & 1)
+    vals = np.where(neg, -1.0, 1.0) / np.sqrt(k)
+    return rows, vals
+
+
+def device_sample_rows(p, d, seed):
+    """The device weighted row sampling (pamd_sample_rows_f64): the CDF of the
+    weights summed in 1024 chunks of ceil(m / 1024) rows (chunk totals, then
+    their running sum, then within each chunk, in order); draw i takes
+    u = u53_open0(x, y) of Philox(counter (i, i >> 32, 0x3C6EF372,
+    0x510E527F), key seed), the first row j with cdf[j] >= u * total, and the
+    scale 1 / sqrt(d p_j / total)."""
+    p = np.asarray(p, dtype=np.float64)
+    m = p.size
+    chunk = -(-m // 1024)
+    cdf = np.empty(m)
+    tots = np.array([sum_in_order(p[min(m, t * chunk):min(m, (t + 1) * chunk)]) for t in range(1024)])
+    run = 0.0
+    starts = np.empty(1024)
+    for t in range(1024):
+        starts[t] = run
+        run += tots[t]
+    for t in range(1024):
+        i0, i1 = min(m, t * chunk), min(m, (t + 1) * chunk)
+        r = starts[t]
+        for i in range(i0, i1):
+            r += p[i]
+            cdf[i] = r
+    total = cdf[m - 1]
+    k0, k1 = _seed_words(seed)
+    i = np.arange(d, dtype=np.uint64)
+    x, y, _, _ = philox4x32_10(i & _M32, i >> np.uint64(32), 0x3C6EF372, 0x510E527F, k0, k1)
+    target = u53_open0(x, y) * total
+    idx = np.searchsorted(cdf, target, side="left")
+    idx = np.minimum(idx, m - 1)
+    scale = 1.0 / np.sqrt(float(d) * p[idx] / total)
+    return idx.astype(np.int64), scale
+
+
+def sum_in_order(v):
+    """Left-to-right FP64 sum (the order of a sequential loop)."""
+    s = 0.0
+    for x in np.asarray(v, dtype=np.float64).tolist():
+        s += x
+    return s

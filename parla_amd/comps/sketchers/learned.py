@@ -268,7 +268,7 @@ class SPOLearned:
     """SPO (least_squares.py:206-369 call contract) with the learned row-sampling
     sketch and the plain SJLT SPO as the fallback."""
 
-    def __init__(self, mlp, d=4096, kappa_max=10.0, mix=0.1, fallback=None, lanczos_steps=12, sampling="device"):
+    def __init__(self, mlp, d=4096, kappa_max=10.0, mix=0.1, fallback=None, lanczos_steps=4, sampling="device"):
         from parla_amd.drivers.least_squares import SPO
         import parla_amd.comps.sketchers.oblivious as sko
         self.mlp = mlp

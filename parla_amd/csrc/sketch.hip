@@ -68,6 +68,37 @@ __global__ __launch_bounds__(1024) void csr_scan_kernel(int64_t n_short, int nbl
   csr_scan_body(n_short, nblk, cnt, rowptr);
 }
 
+// With many column blocks the scan goes through groups of CSR_GB blocks, so
+// that no thread walks all of them: gsum[g][r] = the count of row r in group
+// g; the one-workgroup scan turns gsum into the groups' start offsets; then
+// every (group, row) hands out the offsets of its blocks.  Integer sums: the
+// offsets equal the direct scan's.
+constexpr int CSR_GB = 32;
+
+__global__ __launch_bounds__(256) void csr_group_sum_kernel(int64_t n_short, int nblk, const int32_t* __restrict__ cnt,
+                                                            int32_t* __restrict__ gsum) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= n_short) return;
+  const int b0 = blockIdx.y * CSR_GB, b1 = min(nblk, b0 + CSR_GB);
+  int32_t t = 0;
+  for (int b = b0; b < b1; ++b) t += cnt[(int64_t)b * n_short + r];
+  gsum[(int64_t)blockIdx.y * n_short + r] = t;
+}
+
+__global__ __launch_bounds__(256) void csr_group_offsets_kernel(int64_t n_short, int nblk,
+                                                                const int32_t* __restrict__ gsum,
+                                                                int32_t* __restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= n_short) return;
+  const int b0 = blockIdx.y * CSR_GB, b1 = min(nblk, b0 + CSR_GB);
+  int32_t run = gsum[(int64_t)blockIdx.y * n_short + r];
+  for (int b = b0; b < b1; ++b) {
+    const int32_t t = cnt[(int64_t)b * n_short + r];
+    cnt[(int64_t)b * n_short + r] = run;
+    run += t;
+  }
+}
+
 // Thread t owns rows r with r % 256 == t and places them in column order.
 // One wavefront per block of CSR_CB columns walks its columns in order; the k
 // entries of a column go to their rows' next free slots together (the rows of
@@ -365,7 +396,9 @@ extern "C" int pamd_sjlt_csr_workspace(int64_t n_short, int64_t n_long, int64_t*
   PAMD_ARG_CHECK(n_short >= 1 && n_short <= PAMD_CSR_MAX_SHORT, 1);
   PAMD_ARG_CHECK(n_long >= 0, 2);
   PAMD_ARG_CHECK(bytes != nullptr, 3);
-  *bytes = imax64(1, pamd_cdiv(n_long, CSR_CB)) * n_short * (int64_t)sizeof(int32_t);
+  const int64_t nblk = imax64(1, pamd_cdiv(n_long, CSR_CB));
+  const int64_t ngrp = nblk > CSR_GB ? pamd_cdiv(nblk, CSR_GB) : 0;  // group sums of the two-level scan
+  *bytes = (nblk + ngrp) * n_short * (int64_t)sizeof(int32_t);
   return PAMD_OK;
 }
 
@@ -385,7 +418,18 @@ extern "C" int pamd_sjlt_csr(int64_t n_short, int64_t n_long, int k, const int32
   int32_t* cnt = (int32_t*)workspace;
   csr_count_kernel<<<nblk, 256, n_short * sizeof(int32_t), st>>>(n_short, n_long, k, rows, cnt);
   PAMD_LAUNCH_CHECK();
-  csr_scan_kernel<<<1, 1024, 0, st>>>(n_short, nblk, cnt, rowptr);
+  if (nblk > CSR_GB) {
+    const int ngrp = (int)pamd_cdiv(nblk, CSR_GB);
+    int32_t* gsum = cnt + (int64_t)nblk * n_short;
+    const dim3 grid((unsigned)pamd_cdiv(n_short, 256), (unsigned)ngrp);
+    csr_group_sum_kernel<<<grid, 256, 0, st>>>(n_short, nblk, cnt, gsum);
+    PAMD_LAUNCH_CHECK();
+    csr_scan_kernel<<<1, 1024, 0, st>>>(n_short, ngrp, gsum, rowptr);
+    PAMD_LAUNCH_CHECK();
+    csr_group_offsets_kernel<<<grid, 256, 0, st>>>(n_short, nblk, gsum, cnt);
+  } else {
+    csr_scan_kernel<<<1, 1024, 0, st>>>(n_short, nblk, cnt, rowptr);
+  }
   PAMD_LAUNCH_CHECK();
   if (n_long > 0) {
     csr_fill_kernel<<<nblk, 64, n_short * sizeof(int32_t), st>>>(n_short, n_long, k, rows, vals, cnt, cols, cvals);

@@ -238,6 +238,22 @@ def test_sjlt_csr_and_products_match_scipy(ops, golden, tag):
     assert rel(right, X @ S.T) < 100 * EPS
 
 
+@pytest.mark.parametrize("r,c,k", [(2048, 131072, 8), (266, 4096, 8), (300, 2049, 8), (7, 70000, 4)])
+def test_sjlt_csr_many_column_blocks(ops, r, c, k):
+    """Past 32 blocks of 64 columns the CSR offsets come from the grouped
+    scan: the result must equal scipy's CSC -> CSR of the same draw."""
+    _say(f"sjlt csr {r} x {c}")
+    import oracle as orc
+    rows, vals = orc.device_sjlt_draw(r, c, k, 1234)
+    S = sp.csc_matrix((vals.reshape(-1), rows.reshape(-1), np.arange(0, c * k + 1, k)), shape=(r, c))
+    rowptr, cols, cv = ops.sjlt_csr(r, c, k, dev(rows.reshape(-1)), dev(vals.reshape(-1)))
+    Scsr = S.tocsr()
+    Scsr.sort_indices()
+    np.testing.assert_array_equal(host(rowptr), Scsr.indptr)
+    np.testing.assert_array_equal(host(cols)[: Scsr.nnz], Scsr.indices)
+    np.testing.assert_array_equal(host(cv)[: Scsr.nnz], Scsr.data)
+
+
 def test_sjlt_right_wide_rows_cross_lds_tiles(ops):
     _say("sjlt right n=4096 k=266")
     import oracle as orc

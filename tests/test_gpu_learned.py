@@ -124,7 +124,7 @@ def test_trained_network_gains_over_the_method_without_it(lm):
     p_uniform = torch.full((m,), 1.0 / m, dtype=torch.float64, device=dev)
     rn = torch.sum(F.double() ** 2, dim=1)
     p_rownorm = 0.9 * rn / rn.sum() + 0.1 / m
-    alg = lm.SPOLearned(net, d=d, kappa_max=1e30)
+    alg = lm.SPOLearned(net, d=d, kappa_max=1e30, lanczos_steps=12)
     kap = {}
     for name, p in [("learned", p_learned), ("uniform", p_uniform), ("rownorm", p_rownorm)]:
         R, z, k = alg.sketch_factor(A, b, np.random.default_rng(2), probs=p)

@@ -680,3 +680,66 @@ def test_srct_and_sparse_sign_match_reference(golden):
     for tag, (rr, cc, seed, dens) in {"ss20x60": (20, 60, 7, 0.1), "ss50x8": (50, 8, 8, 0.3)}.items():
         np.testing.assert_array_equal(orc.sparse_sign_operator(rr, cc, np.random.default_rng(seed), dens),
                                       golden[f"sps_{tag}"])
+
+
+# --------------------------------------------------------------------------
+# the device draws' restatement (Philox-4x32-10 streams)
+# --------------------------------------------------------------------------
+
+def test_philox_known_answers():
+    """The known-answer vectors published with Random123 for philox4x32_10
+    (counter, key -> output)."""
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+           ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+           ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+            (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+    for c, k, want in kat:
+        assert tuple(int(x) for x in orc.philox4x32_10(*c, *k)) == want
+    # vectorised counters give the scalar results
+    out = orc.philox4x32_10(np.array([0, 0x243F6A88]), np.array([0, 0x85A308D3]), np.array([0, 0x13198A2E]),
+                            np.array([0, 0x03707344]), 0, 0)
+    assert int(out[0][0]) == 0x6627E8D5
+
+
+def test_u53_range_and_end_points():
+    u = orc.u53_open0(np.array([0, 0xFFFFFFFF, 0x80000000]), np.array([0, 0xFFFFFFFF, 0]))
+    assert u[0] == 2.0 ** -53 and u[1] == 1.0 and u[2] == 0.5 + 2.0 ** -53
+
+
+def test_device_gaussian_fill_restatement():
+    X = orc.device_gaussian_fill(300, 1000, 1.0, 1234)
+    assert abs(X.mean()) < 5 / np.sqrt(X.size) and abs(X.var() - 1) < 6 * np.sqrt(2 / X.size)
+    # fourth moment 3, no correlation between the two halves of a Box-Muller pair
+    assert abs(np.mean(X ** 4) - 3) < 0.1
+    assert abs(np.mean(X.reshape(-1, 2)[:, 0] * X.reshape(-1, 2)[:, 1])) < 5 / np.sqrt(X.size / 2)
+    # an entry depends only on (seed, offset + index): a sub-block with its offset equals the slice
+    np.testing.assert_array_equal(orc.device_gaussian_fill(1, 1000, 1.0, 1234, offset=7000), X[7:8])
+    np.testing.assert_array_equal(orc.device_gaussian_fill(3, 5, 2.0, 1234, offset=1), 2 * X.reshape(-1)[1:16].reshape(3, 5))
+
+
+def test_device_sjlt_draw_restatement():
+    rows, vals = orc.device_sjlt_draw(30, 2000, 8, 99)
+    assert rows.min() >= 0 and rows.max() < 30
+    assert all(len(set(r)) == 8 for r in rows.tolist())  # distinct rows per column
+    assert np.all(np.abs(np.abs(vals) - 1 / np.sqrt(8)) == 0)
+    assert abs(np.mean(vals < 0) - 0.5) < 5 * np.sqrt(0.25 / vals.size)
+    counts = np.bincount(rows.reshape(-1), minlength=30)
+    assert np.all(np.abs(counts - counts.mean()) < 6 * np.sqrt(counts.mean()))
+    # with replacement when the short side is below k
+    rows2, _ = orc.device_sjlt_draw(3, 50, 8, 5)
+    assert rows2.max() < 3
+
+
+def test_device_sample_rows_restatement():
+    rng = np.random.default_rng(4)
+    p = rng.random(3000)
+    p[::7] = 0.0  # zero weights are never drawn
+    idx, scale = orc.device_sample_rows(p, 400000, 77)
+    assert np.all(p[idx] > 0)
+    freq = np.bincount(idx, minlength=p.size) / idx.size
+    q = p / p.sum()
+    assert np.all(np.abs(freq - q) <= 6 * np.sqrt(q / idx.size) + 1e-12)
+    np.testing.assert_allclose(scale, 1 / np.sqrt(idx.size * q[idx]), rtol=1e-13)
+    # the chunked CDF equals the plain cumulative sum to rounding
+    idx1, _ = orc.device_sample_rows(np.ones(5), 10000, 3)
+    assert set(np.unique(idx1)) == set(range(5))
