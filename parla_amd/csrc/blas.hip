@@ -223,13 +223,13 @@ int gemm_splits(int64_t M, int64_t N, int64_t K) {
     return (int)imax64(1, imin64(s, 64));
   }
   // a large output with a long K (the power iteration's A S, A^T Y and Q^T A
-  // at 8192 x 4096, k = 266): split K until there are about 3072 workgroups,
-  // twelve per CU, so enough wavefronts are resident to hide MFMA and load
-  // latency (measured best of 2048 / 3072 / 4096); the FP64 partials are
-  // cheap next to the K reduction
+  // at 8192 x 4096, k = 266): split K until there are about 1536 workgroups,
+  // six per CU.  Kernel plus reduction of the partials, interleaved medians:
+  // 1024 / 1536 / 2048 / 3072 workgroups took 436 / 398 / 439 / 415 us
+  // (A^T Y), 432 / 400 / 412 / 415 (A S) and 535 / 427 / 484 / 445 (Q^T A)
   if (M * N >= (1LL << 20) && K >= 2048) {
-    if (tiles >= 3072) return 1;
-    const int64_t s = imin64(pamd_cdiv(3072, tiles), K / 256);
+    if (tiles >= 1536) return 1;
+    const int64_t s = imin64(pamd_cdiv(1536, tiles), K / 256);
     return (int)imax64(1, imin64(s, 64));
   }
   // small outputs: about 1024 workgroups, at least 256 K per slice
