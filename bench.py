@@ -180,7 +180,7 @@ def main():
         del data
         A5, _ = lm.family_matrix(m5, n5, 777, dev)
         rng5 = np.random.default_rng(5)
-        b5 = A5 @ torch.from_numpy(rng5.standard_normal(n5)).to(dev) + \
+        b5 = A5.double() @ torch.from_numpy(rng5.standard_normal(n5)).to(dev) + \
             1e-6 * torch.from_numpy(rng5.standard_normal(m5)).to(dev)
         learned = lm.SPOLearned(net, d=d5, kappa_max=30.0)
         plain5 = pa.SPO(pa.SkOpSJ(8, backend="device"), 4)
@@ -413,7 +413,7 @@ def main():
         lev5 = torch.exp(lm.leverage_labels(DG5).double()) * n5 / m5
         del DG5
         rng5 = np.random.default_rng(5)
-        b5 = A5 @ torch.from_numpy(rng5.standard_normal(n5)).to(dev) + \
+        b5 = A5.double() @ torch.from_numpy(rng5.standard_normal(n5)).to(dev) + \
             1e-6 * torch.from_numpy(rng5.standard_normal(m5)).to(dev)
         G5 = lm.presketch(n5, 4242, dev)
         F5 = lm.features(A5, G5)
@@ -443,7 +443,8 @@ def main():
         x_s5 = out["S5"][0]
         _, lgs5 = plain5(A5, b5, 0.0, 1e-12, 200, 7, logging=True)
         learned_extra = {
-            "learned_workload": f"A {m5}x{n5} (Gaussian rows, 1% x100, right-scaled to cond 1e6), features A G16, "
+            "learned_workload": f"A {m5}x{n5} FP32 (Gaussian rows, 1% x100, right-scaled to cond 1e6; the FP64 "
+                                f"solvers widen it inside the timed solve), features A G16, "
                                 f"MLP 16-64-64-1 BF16 weights, row sampling d={d5}, SJLT fallback above kappa 30",
             "learned_training": {"matrices": args.train_matrices, "rows": int(rows_trained),
                                  "data_s": round(t_data, 2), "train_s": round(t_train, 2),

@@ -501,7 +501,7 @@ int pamd_potrf_batched_f64(int64_t nprob, const int64_t* n, const int64_t* off,
                            double* pool, int* info, void* stream);
 int pamd_trtri_batched_f64(int64_t nprob, int64_t nmax, const int64_t* n,
                            const int64_t* off, const double* R, double* Rinv,
-                           void* stream);
+                           double* RinvT, void* stream);  /* RinvT: the transposes, or NULL */
 
 /* ------------------------------------------------------------------------ */
 /* Batched least squares over problems of uneven size                        */
@@ -552,6 +552,17 @@ int pamd_bt_reduce(int64_t nprob, const pamd_ls_desc* descs, const double* part,
  * the preconditioner solves of comps/preconditioning.py:28,37, batched. */
 int pamd_bt_trsv(int64_t nprob, const pamd_ls_desc* descs, int trans,
                  const double* R, double* v, void* stream);
+
+/* v_i <- Rinv_i v_i (trans 0) or Rinv_i^T v_i (trans 1) for every problem,
+ * Rinv_i = R_i^{-1} (from pamd_trtri_batched_f64, laid out like R) and
+ * RinvT_i its transpose (both read as coalesced row segments): the same
+ * preconditioner (comps/preconditioning.py:28,37) applied as a product, for
+ * the refinement's inner CG, which applies it twice per step.  The iteration
+ * then runs on the fixed operator Rinv_i^T A_i^T A_i Rinv_i; the rounding of
+ * Rinv_i (of order u kappa(R_i)) changes the preconditioner, not the normal
+ * equations the refinement solves. */
+int pamd_bt_trmv(int64_t nprob, const pamd_ls_desc* descs, int trans,
+                 const double* Rinv, const double* RinvT, double* v, void* stream);
 
 /* Conjugate gradients on every problem's preconditioned normal equations,
  * recurrences on the device; state[4 i] = r.r, state[4 i + 1] = 1 while

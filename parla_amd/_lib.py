@@ -89,11 +89,12 @@ SIGNATURES = {
     "pamd_residual_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_nrm2_workspace": (c_int, [c_i64, ctypes.POINTER(c_i64)]),
     "pamd_potrf_batched_f64": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp, c_vp]),
-    "pamd_trtri_batched_f64": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "pamd_trtri_batched_f64": (c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_normal": (c_int, [c_int, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_residual": (c_int, [c_int, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_reduce": (c_int, [c_i64, c_vp, c_vp, c_int, c_vp, c_vp, c_vp]),
     "pamd_bt_trsv": (c_int, [c_i64, c_vp, c_int, c_vp, c_vp, c_vp]),
+    "pamd_bt_trmv": (c_int, [c_i64, c_vp, c_int, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_cg": (c_int, [c_i64, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_axpy": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_bt_copy": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp]),

@@ -134,7 +134,7 @@ def presketch(n, seed, dev):
 
 def features(A, G):
     """F = A G normalised to unit root-mean-square, FP32 (m x 16)."""
-    F = ops.gemm(A, G)
+    F = ops.gemm(ops.as_device(A), G)
     rms = torch.sqrt(torch.mean(F * F))
     return (F / rms).to(torch.float32).contiguous()
 
@@ -159,7 +159,8 @@ def leverage_labels(A, rounds=2):
 
 def family_matrix(m, n, seed, dev, heavy_frac=0.01, heavy_scale=100.0, cond=1e6):
     """A = D G diag(s) V^T: Gaussian rows G (device Philox), D scales a seeded
-    1% of rows by 100, s log-spaced from 1 to 1/cond, V Haar.  Returns
+    1% of rows by 100, s log-spaced from 1 to 1/cond, V Haar; A is returned in
+    FP32, the family's data type (the FP64 solvers widen it exactly).  Returns
     (A, D G): leverage scores do not depend on the right factor, so labels
     come from the better-conditioned D G."""
     rng = np.random.default_rng(seed)
@@ -168,7 +169,7 @@ def family_matrix(m, n, seed, dev, heavy_frac=0.01, heavy_scale=100.0, cond=1e6)
     G[heavy] *= heavy_scale
     V = np.linalg.qr(rng.standard_normal((n, n)))[0]
     M = torch.from_numpy(np.ascontiguousarray(np.logspace(0, -np.log10(cond), n)[:, None] * V.T)).to(dev)
-    return ops.gemm(G, M), G
+    return ops.gemm(G, M).to(torch.float32), G
 
 
 def training_set(num, m, n, seed, dev, log=None):
@@ -234,16 +235,26 @@ def row_gather(A, idx, scale, out):
 
 def kappa_estimate(A, R, steps=12, seed=0):
     """Estimate kappa(A R^{-1}) from the Lanczos tridiagonal that CG on
-    R^{-T} A^T A R^{-1} builds (extreme Ritz values); steps passes over A."""
+    R^{-T} A^T A R^{-1} builds (extreme Ritz values); steps passes over A.
+    R^{-1} is formed once (the estimate needs only its leading digits), so a
+    step is one normal pass and two products with it instead of two
+    triangular solves."""
     n = R.shape[0]
     normal = ops.NormalOp(A)
+    Ri = ops.trtri(R) if n <= 2048 else None
+
+    def apply(v):
+        if Ri is not None:
+            return ops.gemv(Ri, normal(ops.gemv(Ri, v)), trans=True)
+        return ops.trsv_(R, normal(ops.trsv_(R, v.clone())), trans=True)
+
     rng = np.random.default_rng(seed)
     r = torch.from_numpy(rng.standard_normal(n)).to(A.device)
     p = r.clone()
     rr = torch.dot(r, r)
     alphas, betas = [], []
     for _ in range(steps):
-        q = ops.trsv_(R, normal(ops.trsv_(R, p.clone())), trans=True)
+        q = apply(p)
         alpha = rr / torch.dot(p, q)
         r = r - alpha * q
         rr_new = torch.dot(r, r)
@@ -282,6 +293,7 @@ class SPOLearned:
 
     def sketch_factor(self, At, bt, rng, probs=None):
         """Row-sampling sketch of [A | b] and its QR; returns (R, z, kappa_est)."""
+        At = ops.as_device(At)
         m, n = At.shape
         dev = At.device
         if probs is None:

@@ -286,6 +286,53 @@ __global__ __launch_bounds__(256) void bt_trsv_kernel(const Desc* __restrict__ D
   for (int i = threadIdx.x; i < n; i += blockDim.x) v[d.nv_off + i] = xs[i];
 }
 
+// v_i <- M_i^T v_i, M_i = R_i^{-1} (upper triangular) or its transpose
+// (lower), row-major with ld n_i, so that both R^{-1} v and R^{-T} v are a
+// product with a transposed matrix: one workgroup of four wavefronts per
+// problem, each lane owning eight columns, each wavefront every fourth row
+// (two rows' coalesced segments loaded at a time), the wavefronts' column
+// sums added in wavefront order.
+__global__ __launch_bounds__(256) void bt_trmv_kernel(const Desc* __restrict__ D, int lower,
+                                                      const double* __restrict__ Mp, double* __restrict__ v) {
+  __shared__ double xs[BT_MAXN];
+  __shared__ double red[4][BT_MAXN];
+  const Desc d = D[blockIdx.x];
+  const int n = (int)d.n;
+  const double* M = Mp + d.r_off;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) xs[i] = v[d.nv_off + i];
+  __syncthreads();
+  double acc[BT_MAXN / 64];
+#pragma unroll
+  for (int k = 0; k < BT_MAXN / 64; ++k) acc[k] = 0.0;
+  for (int i0 = wave; i0 < n; i0 += 8) {
+    double mv[2][BT_MAXN / 64];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = i0 + 4 * h;
+      const double* row = M + (int64_t)(i < n ? i : 0) * n;
+#pragma unroll
+      for (int k = 0; k < BT_MAXN / 64; ++k) {
+        const int j = lane + 64 * k;
+        const bool in = i < n && j < n && (lower ? (j <= i) : (j >= i));
+        mv[h][k] = in ? row[j] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = i0 + 4 * h;
+      const double xi = (i < n) ? xs[i] : 0.0;
+#pragma unroll
+      for (int k = 0; k < BT_MAXN / 64; ++k) acc[k] += mv[h][k] * xi;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < BT_MAXN / 64; ++k) red[wave][lane + 64 * k] = acc[k];
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x)
+    v[d.nv_off + j] = ((red[0][j] + red[1][j]) + red[2][j]) + red[3][j];
+}
+
 __device__ inline double bt_block_dot(const double* a, const double* b, int64_t n, double* red) {
   double s = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += a[i] * b[i];
@@ -798,6 +845,22 @@ extern "C" int pamd_bt_trsv(int64_t nprob, const pamd_ls_desc* descs, int trans,
   if (nprob == 0) return PAMD_OK;
   const size_t shmem = (size_t)(BT_MAXN + TRSV_SCRATCH_DOUBLES) * sizeof(double);
   bt_trsv_kernel<<<(unsigned)nprob, 256, shmem, (hipStream_t)stream>>>((const Desc*)descs, trans, R, v);
+  PAMD_LAUNCH_CHECK();
+  return PAMD_OK;
+}
+
+extern "C" int pamd_bt_trmv(int64_t nprob, const pamd_ls_desc* descs, int trans, const double* Rinv,
+                            const double* RinvT, double* v, void* stream) {
+  PAMD_ARG_CHECK(nprob >= 0, 1);
+  PAMD_ARG_CHECK(descs != nullptr || nprob == 0, 2);
+  PAMD_ARG_CHECK(trans == 0 || trans == 1, 3);
+  PAMD_ARG_CHECK(Rinv != nullptr || nprob == 0, 4);
+  PAMD_ARG_CHECK(RinvT != nullptr || nprob == 0, 5);
+  PAMD_ARG_CHECK(v != nullptr || nprob == 0, 6);
+  if (nprob == 0) return PAMD_OK;
+  // R^{-T} v = Rinv^T v; R^{-1} v = (Rinv^T)^T v
+  bt_trmv_kernel<<<(unsigned)nprob, 256, 0, (hipStream_t)stream>>>((const Desc*)descs, trans ? 0 : 1,
+                                                                   trans ? Rinv : RinvT, v);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

@@ -452,7 +452,8 @@ __global__ __launch_bounds__(256) void trtri_kernel(int64_t n, const double* __r
 // batched R^{-1}: grid (max n, problems); column j of problem i when j < n[i]
 __global__ __launch_bounds__(256) void trtri_batched_kernel(const int64_t* __restrict__ nv,
                                                             const int64_t* __restrict__ off,
-                                                            const double* __restrict__ Rp, double* __restrict__ Rip) {
+                                                            const double* __restrict__ Rp, double* __restrict__ Rip,
+                                                            double* __restrict__ RiTp) {
   extern __shared__ double smem[];
   const int64_t n = nv[blockIdx.y];
   const int64_t j = blockIdx.x;
@@ -465,6 +466,10 @@ __global__ __launch_bounds__(256) void trtri_batched_kernel(const int64_t* __res
   __syncthreads();
   trsv_upper_block<8>(0, sz, R, n, xs, smem + sz);
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Ri[i * n + j] = (i <= j) ? xs[i] : 0.0;
+  if (RiTp != nullptr) {  // row j of the transpose: contiguous
+    double* RiT = RiTp + off[blockIdx.y];
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) RiT[j * n + i] = (i <= j) ? xs[i] : 0.0;
+  }
 }
 
 __global__ __launch_bounds__(64) void trsm_dinv_kernel(int k, const double* __restrict__ R, int64_t ldr,
@@ -705,7 +710,7 @@ extern "C" int pamd_potrf_batched_f64(int64_t nprob, const int64_t* n, const int
 }
 
 extern "C" int pamd_trtri_batched_f64(int64_t nprob, int64_t nmax, const int64_t* n, const int64_t* off,
-                                      const double* R, double* Rinv, void* stream) {
+                                      const double* R, double* Rinv, double* RinvT, void* stream) {
   PAMD_ARG_CHECK(nprob >= 0, 1);
   PAMD_ARG_CHECK(nmax >= 0 && nmax <= PAMD_TRSV_MAX_N, 2);
   PAMD_ARG_CHECK(n != nullptr || nprob == 0, 3);
@@ -715,7 +720,7 @@ extern "C" int pamd_trtri_batched_f64(int64_t nprob, int64_t nmax, const int64_t
   if (nprob == 0 || nmax == 0) return PAMD_OK;
   const size_t shmem = (size_t)(nmax + TRSV_SCRATCH_DOUBLES) * sizeof(double);
   dim3 grid((unsigned)nmax, (unsigned)nprob);
-  trtri_batched_kernel<<<grid, 256, shmem, (hipStream_t)stream>>>(n, off, R, Rinv);
+  trtri_batched_kernel<<<grid, 256, shmem, (hipStream_t)stream>>>(n, off, R, Rinv, RinvT);
   PAMD_LAUNCH_CHECK();
   return PAMD_OK;
 }

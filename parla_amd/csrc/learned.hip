@@ -374,23 +374,43 @@ __global__ __launch_bounds__(1024) void weights_cdf_kernel(int64_t m, const doub
   const int t = threadIdx.x;
   const int64_t chunk = pamd_cdiv(m, 1024);
   const int64_t i0 = imin64(m, (int64_t)t * chunk), i1 = imin64(m, i0 + chunk);
+  // the chunk's weights eight at a time (loads issued together, added in order)
   double s = 0.0;
-  for (int64_t i = i0; i < i1; ++i) s += p[i];
+  for (int64_t i = i0; i < i1; i += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = (i + u < i1) ? p[i + u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i + u < i1) s += v[u];
+  }
   tot[t] = s;
   __syncthreads();
   if (t == 0) {
     double run = 0.0;
-    for (int k = 0; k < 1024; ++k) {
-      const double v = tot[k];
-      tot[k] = run;
-      run += v;
+    for (int k = 0; k < 1024; k += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = tot[k + u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        tot[k + u] = run;
+        run += v[u];
+      }
     }
   }
   __syncthreads();
   double run = tot[t];
-  for (int64_t i = i0; i < i1; ++i) {
-    run += p[i];
-    cdf[i] = run;
+  for (int64_t i = i0; i < i1; i += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = (i + u < i1) ? p[i + u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i + u < i1) {
+        run += v[u];
+        cdf[i + u] = run;
+      }
   }
 }
 

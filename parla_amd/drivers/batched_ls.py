@@ -180,6 +180,8 @@ class Batch:
         f64 = dict(dtype=ops.F64, device=dev)
         self.x, self.z, self.r, self.p, self.q, self.t, self.g, self.rhs = (torch.zeros(NV, **f64) for _ in range(8))
         self.R = torch.zeros(int(r_off[-1]), **f64)
+        self.Rinv = torch.zeros(int(r_off[-1]), **f64)
+        self.RinvT = torch.zeros(int(r_off[-1]), **f64)
         self.part = torch.zeros(self.totals["part"], **f64)
         self.rss = torch.zeros(P, **f64)
         self.gg = torch.zeros(P, **f64)
@@ -269,6 +271,10 @@ class BatchedSPO:
         bad = info.view(3, P).ne(0).any(0).cpu().numpy()
         if bad.any():
             raise ArithmeticError(f"batched CholeskyQR3 failed for problems {np.nonzero(bad)[0][:8].tolist()}")
+        # R_i^{-1} and its transpose once: the refinement applies the
+        # preconditioner as products with them (two per CG step)
+        ops.check(L.pamd_trtri_batched_f64(P, nmax, ops._p(B.n_dev), ops._p(B.roff_dev), ops._p(B.R),
+                                           ops._p(B.Rinv), ops._p(B.RinvT), st), "pamd_trtri_batched_f64")
 
     # -- batched passes ------------------------------------------------------
     def _pass(self, B, kind):
@@ -302,8 +308,9 @@ class BatchedSPO:
                                            ops.stream(B.dev)), "pamd_bt_reduce")
 
     def _trsv(self, B, v, trans):
-        ops.check(ops.lib().pamd_bt_trsv(B.P, ops._p(B.desc), 1 if trans else 0, ops._p(B.R), ops._p(v),
-                                         ops.stream(B.dev)), "pamd_bt_trsv")
+        """v_i <- R_i^{-1} v_i (R_i^{-T} v_i), as products with R_i^{-1} (formed in factor)."""
+        ops.check(ops.lib().pamd_bt_trmv(B.P, ops._p(B.desc), 1 if trans else 0, ops._p(B.Rinv), ops._p(B.RinvT),
+                                         ops._p(v), ops.stream(B.dev)), "pamd_bt_trmv")
 
     def _copy(self, B, a, v):
         ops.check(ops.lib().pamd_bt_copy(B.P, ops._p(B.desc), ops._p(a), ops._p(v), ops.stream(B.dev)),

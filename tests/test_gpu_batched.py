@@ -80,7 +80,9 @@ def test_batched_passes_match_numpy_and_ignore_launch_shape(bl):
 
 
 def test_batched_trsv_and_factors(bl):
-    _say("batched factor + trsv")
+    _say("batched factor + preconditioner products + trsv")
+    import scipy.linalg as la
+    from parla_amd import ops
     As, bs, _ = _problems(9, 3)
     dev = torch.device("cuda")
     B = bl.Batch(As, bs, dev)
@@ -88,18 +90,30 @@ def test_batched_trsv_and_factors(bl):
     solver.factor(B, np.random.default_rng(4))
     rng = np.random.default_rng(5)
     v = torch.from_numpy(rng.standard_normal(B.t.numel())).to(dev)
+    u = 2.0 ** -53
+    for trans in (False, True):
+        B.t.copy_(v)
+        solver._trsv(B, B.t, trans)  # products with the formed R_i^{-1} (or its transpose)
+        for i, A in enumerate(As):
+            R = B.R_view(i).cpu().numpy()
+            assert np.allclose(R, np.triu(R))
+            if not trans:  # R preconditions A: kappa(A R^-1) small
+                s = np.linalg.svd(A.astype(np.float64) @ np.linalg.inv(R), compute_uv=False)
+                assert s[0] / s[-1] < 10, (i, s[0] / s[-1])
+            # to the accuracy of the inverse: ||x - R^{-1} v|| <= c n u kappa(R) ||R^{-1} v||
+            x = B.vec(B.t, i).cpu().numpy()
+            xr = la.solve_triangular(R, B.vec(v, i).cpu().numpy(), trans=1 if trans else 0)
+            n = R.shape[0]
+            assert np.linalg.norm(x - xr) <= 10 * n * u * np.linalg.cond(R) * np.linalg.norm(xr), (i, trans)
+    # the batched triangular solve entry point: componentwise backward stable
     B.t.copy_(v)
-    solver._trsv(B, B.t, False)
+    ops.check(ops.lib().pamd_bt_trsv(B.P, ops._p(B.desc), 0, ops._p(B.R), ops._p(B.t), ops.stream(dev)),
+              "pamd_bt_trsv")
     for i, A in enumerate(As):
         R = B.R_view(i).cpu().numpy()
-        assert np.allclose(R, np.triu(R))
-        # R preconditions A: kappa(A R^-1) small
-        s = np.linalg.svd(A.astype(np.float64) @ np.linalg.inv(R), compute_uv=False)
-        assert s[0] / s[-1] < 10, (i, s[0] / s[-1])
-        # the solve is backward stable: |R x - v| <= c u |R| |x| componentwise
         x = B.vec(B.t, i).cpu().numpy()
         res = np.abs(R @ x - B.vec(v, i).cpu().numpy())
-        assert (res <= 1e3 * 2.0 ** -53 * (np.abs(np.triu(R)) @ np.abs(x)) + 1e-300).all(), i
+        assert (res <= 1e3 * u * (np.abs(np.triu(R)) @ np.abs(x)) + 1e-300).all(), i
 
 
 def test_graph_replays_equal_eager(bl):

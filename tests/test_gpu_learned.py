@@ -117,7 +117,8 @@ def test_trained_network_gains_over_the_method_without_it(lm):
     A, DG = lm.family_matrix(m, n, 999, dev)
     lev = torch.exp(lm.leverage_labels(DG).double()) * n / m
     rng = np.random.default_rng(1)
-    b = A @ torch.from_numpy(rng.standard_normal(n)).to(dev) + 1e-6 * torch.from_numpy(rng.standard_normal(m)).to(dev)
+    assert A.dtype == torch.float32  # the family's data type
+    b = A.double() @ torch.from_numpy(rng.standard_normal(n)).to(dev) + 1e-6 * torch.from_numpy(rng.standard_normal(m)).to(dev)
     G = lm.presketch(n, 4242, dev)
     F = lm.features(A, G)
     p_learned = lm.sampling_probabilities(net.predict(F), 0.1, n)
