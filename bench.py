@@ -114,6 +114,9 @@ def main():
                     help="only run the two BF16 MFMA GEMMs once each: the split GEMM on the QB1 shape "
                          "A S (8192 x 266, K 4096, 2 pieces) and the exact sketch at A 262144 x 1024 "
                          "(d 4096, 4 pieces) (the workload of the rocprofv3 SQ counter passes)")
+    ap.add_argument("--profile-qr", action="store_true",
+                    help="only time the sketch factorisations: blocked Householder (geqrf) and the "
+                         "communication-avoiding R factor (pamd_qr_r_f64), 5 calls each on the LS sketch shapes")
     ap.add_argument("--profile-ls", action="store_true",
                     help="only run 10 plain FP64 SPO solves on A 20000 x 200 (the workload of the LS kernel trace)")
     ap.add_argument("--profile-batched", action="store_true",
@@ -162,6 +165,26 @@ def main():
         torch.cuda.synchronize()
         return
 
+    if args.profile_qr:
+        # the sketches [S A | S b] of the LS workloads: C1 (800 x 201), the SJLT
+        # and learned sketches of A 131072 x 512 (2048 / 4096 x 513), C3 (4096 x 1025)
+        for (mq, nq) in [(800, 200), (2048, 512), (4096, 512), (4096, 1024)]:
+            Mq = torch.randn(mq, nq + 1, dtype=torch.float64, device=dev)
+            res = {}
+            for name, fn in [("geqrf", ops.geqrf_), ("qr_r", ops.qr_r_caqr_)]:
+                ts = []
+                for it in range(6):
+                    M = Mq.clone()
+                    torch.cuda.synchronize()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    fn(M, nq)
+                    e1.record()
+                    torch.cuda.synchronize()
+                    ts.append(e0.elapsed_time(e1))
+                res[name] = float(np.median(ts[1:]))
+            log(f"profile-qr: {mq} x {nq + 1}: geqrf {res['geqrf']:.3f} ms, qr_r {res['qr_r']:.3f} ms")
+        return
     if args.profile_ls:
         Al_h, bl_h, _ = mm.config_ls_plain()
         Al, bl = torch.from_numpy(Al_h).to(dev), torch.from_numpy(bl_h).to(dev)

@@ -195,6 +195,26 @@ int pamd_lu_pl_f64(int64_t m, int64_t n, const double* A, int64_t lda,
 int pamd_qr_singular_flag(const void* workspace, int64_t m, int64_t n,
                           int64_t ncols, int** flag_ptr);
 
+/* Workspace (bytes) for pamd_qr_r_f64 on an m x ncols matrix whose first n
+ * columns are factored. */
+int pamd_qr_r_workspace(int64_t m, int64_t n, int64_t ncols, int64_t* bytes);
+
+/* R factor of the first n columns of A (m x ncols, n <= m <= 8192) by
+ * communication-avoiding Householder QR: on exit R, with a non-negative
+ * diagonal, is in the upper triangle of A[:n, :n] and A[:n, n:ncols] holds
+ * the first n rows of Q^T times the input columns (so appending S*b gives
+ * z = Q^T S b in A[:n, n]).  The rest of A is overwritten; Q is not kept.
+ * Replaces la.qr at drivers/least_squares.py:311 with the Q[:d].T @ b_ske of
+ * :315 fused, whose callers use R and z only; R differs from LAPACK's by the
+ * signs of its rows.  Returns PAMD_ERR_UNSUPPORTED for m > 8192 (use
+ * pamd_geqrf_f64 there). */
+int pamd_qr_r_f64(int64_t m, int64_t n, int64_t ncols, double* A, int64_t lda,
+                  void* workspace, void* stream);
+
+/* Pointer (device, int32) to the flag pamd_qr_r_f64 sets when R has an exact
+ * zero on its diagonal (inside `workspace`; host call, no GPU work). */
+int pamd_qr_r_singular_flag(const void* workspace, int64_t m, int** flag_ptr);
+
 /* out = A diag(s) (invert = 0) or A diag(s)^{-1} (invert = 1), A rows x cols
  * (out may alias A).  Forms the SVD right preconditioner M = V / sigma of
  * comps/preconditioning.py:77 and its column-scaled variants

@@ -68,6 +68,9 @@ SIGNATURES = {
     "pamd_geqrf_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp]),
     "pamd_orgqr_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp]),
     "pamd_qr_singular_flag": (c_int, [c_vp, c_i64, c_i64, c_i64, ctypes.POINTER(c_vp)]),
+    "pamd_qr_r_workspace": (c_int, [c_i64, c_i64, c_i64, ctypes.POINTER(c_i64)]),
+    "pamd_qr_r_f64": (c_int, [c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp]),
+    "pamd_qr_r_singular_flag": (c_int, [c_vp, c_i64, ctypes.POINTER(c_vp)]),
     "pamd_lu_workspace": (c_int, [c_i64, c_i64, ctypes.POINTER(c_i64)]),
     "pamd_lu_pl_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
     "pamd_potrf_upper_f64": (c_int, [c_i64, c_vp, c_i64, c_vp, c_vp]),

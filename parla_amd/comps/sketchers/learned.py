@@ -305,7 +305,7 @@ class SPOLearned:
         it = idx if isinstance(idx, torch.Tensor) else torch.from_numpy(idx).to(dev)
         st = scale if isinstance(scale, torch.Tensor) else torch.from_numpy(scale).to(dev)
         row_gather(bt.reshape(-1, 1), it, st, Mx[:, n:])
-        fac = ops.geqrf_(Mx, n)
+        fac = ops.qr_r_(Mx, n)
         R = torch.triu(fac.R_view).contiguous()
         z = Mx[:n, n].clone()
         return R, z, kappa_estimate(At, R, self.lanczos_steps)

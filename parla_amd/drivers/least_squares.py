@@ -114,7 +114,7 @@ class SSO1(OverLstsqSolver):
             Mx[d:].zero_()
             Mx[d:, :n_cols].diagonal().fill_(delta ** 0.5)
         if self.lapack_driver == 'gelsy':
-            ops.geqrf_(Mx, n_cols)
+            ops.qr_r_(Mx, n_cols)
             x = ops.trsv_(Mx[:n_cols, :n_cols], Mx[:n_cols, n_cols].clone())
         else:
             U, sig, Vh = ops.thin_svd(Mx[:, :n_cols])
@@ -204,8 +204,8 @@ class SPO(OverLstsqSolver):
                 Mx[d:, :n_cols].diagonal().fill_(sqrt_delta)
             _sketch_into(S, bt, b_ske)
             Mx[:d, n_cols].copy_(b_ske)
-            fac = ops.geqrf_(Mx, n_cols)
-            # only the upper triangle of this view is R (the reflectors sit below); every
+            fac = ops.qr_r_(Mx, n_cols)
+            # only the upper triangle of this view is R (scratch sits below); every
             # consumer (triangular solves, LSQR) reads the upper triangle only
             R = fac.R_view
             log.time_factor = clock() - tic

@@ -107,7 +107,7 @@ class SPOMixed(OverLstsqSolver):
         log.time_sketch = clock() - tic
 
         tic = clock()
-        fac = ops.geqrf_(Mx, n)
+        fac = ops.qr_r_(Mx, n)
         R = torch.triu(fac.R_view).contiguous()
         z = Mx[:n, n].clone()
         B = At.clone()

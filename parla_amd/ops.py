@@ -268,6 +268,71 @@ def geqrf_(M, n=None):
     return QRFactors(M, n, ws)
 
 
+# pamd_qr_r_f64 takes 1024 < m <= 8192; at m <= 1024 the one-workgroup
+# Householder panels of geqrf are faster than its two-level block/stack steps
+QR_R_MIN_ROWS, QR_R_MAX_ROWS = 1025, 8192
+
+
+class RFactor:
+    """Result of qr_r_: R in the upper triangle of M[:n, :n] and the first n
+    rows of Q^T times the appended columns in M[:n, n:].  R has a
+    non-negative diagonal when pamd_qr_r_f64 factored M, LAPACK's signs when
+    geqrf did (`householder` holds its factors then)."""
+
+    def __init__(self, M, n, ws, householder):
+        self.M = M
+        self.m = M.shape[0]
+        self.n = n
+        self.ws = ws
+        self.householder = householder
+
+    @property
+    def R_view(self):
+        """n x n view whose upper triangle is R (the rest is scratch)."""
+        return self.M[: self.n, : self.n]
+
+    def R(self):
+        return torch.triu(self.R_view)
+
+    def singular(self):
+        if self.householder is not None:
+            return self.householder.singular()
+        flag = ctypes.c_void_p(0)
+        check(lib().pamd_qr_r_singular_flag(_p(self.ws), self.m, ctypes.byref(flag)), "pamd_qr_r_singular_flag")
+        off = flag.value - self.ws.data_ptr()
+        return bool(self.ws[off:off + 4].view(torch.int32).item())
+
+
+def qr_r_(M, n=None):
+    """R factor of the first n columns of M (m x ncols, contiguous), in place,
+    with Q^T applied to columns n..; Q is not formed.  Communication-avoiding
+    Householder QR (pamd_qr_r_f64) for QR_R_MIN_ROWS <= m <= QR_R_MAX_ROWS,
+    blocked Householder (geqrf) otherwise; R's rows may differ from LAPACK's
+    in sign."""
+    if not M.is_contiguous():
+        raise ValueError("qr_r_ needs a contiguous matrix")
+    m, ncols = M.shape
+    n = ncols if n is None else n
+    if n > m:
+        raise ValueError("qr_r_ needs m >= n")
+    if not QR_R_MIN_ROWS <= m <= QR_R_MAX_ROWS:
+        return RFactor(M, n, None, geqrf_(M, n))
+    dev = M.device
+    ws = workspace(_query(lib().pamd_qr_r_workspace, m, n, ncols), dev)
+    check(lib().pamd_qr_r_f64(m, n, ncols, _p(M), max(ncols, 1), _p(ws), stream(dev)), "pamd_qr_r_f64")
+    return RFactor(M, n, ws, None)
+
+
+def qr_r_caqr_(M, n=None):
+    """pamd_qr_r_f64 on any m <= QR_R_MAX_ROWS (what qr_r_ routes to for tall
+    matrices); for tests and measurements of the kernel itself."""
+    m, ncols = M.shape
+    n = ncols if n is None else n
+    ws = workspace(_query(lib().pamd_qr_r_workspace, m, n, ncols), M.device)
+    check(lib().pamd_qr_r_f64(m, n, ncols, _p(M), max(ncols, 1), _p(ws), stream(M.device)), "pamd_qr_r_f64")
+    return RFactor(M, n, ws, None)
+
+
 def qr_q(Y):
     """Economic Q of Y (m x n): la.qr(Y, mode='economic')[0], Q is m x min(m, n)."""
     M = Y.clone(memory_format=torch.contiguous_format)
