@@ -65,7 +65,7 @@ __host__ __device__ inline int lsqr_grid(int64_t m, int64_t n) {
 }
 
 struct LsqrLayout {
-  int64_t st, u, v, w, work, y, pv, pb, xs, total;
+  int64_t st, u, v, w, work, y, pv, pb, xs, dinv, total;
 };
 
 __host__ __device__ inline LsqrLayout lsqr_layout(int64_t m, int64_t n) {
@@ -80,7 +80,8 @@ __host__ __device__ inline LsqrLayout lsqr_layout(int64_t m, int64_t n) {
   L.pv = (L.y + n + 31) / 32 * 32;
   L.pb = L.pv + (int64_t)G * n;
   L.xs = (L.pb + (int64_t)G * 2 + 31) / 32 * 32;
-  L.total = L.xs + (n > LSQR_LDS_MAX_N ? n : 0) + 32;
+  L.dinv = (L.xs + (n > LSQR_LDS_MAX_N ? n : 0) + 31) / 32 * 32;  // inverses of R's 8 x 8 diagonal sub-blocks
+  L.total = L.dinv + pamd_cdiv(n, 8) * 64 + 32;
   return L;
 }
 
@@ -328,7 +329,7 @@ __device__ inline double apply_adjoint(const LsqrConst& K, const double* __restr
   if (K.has_r) {
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) xs[i] = y[i] * inv;
     __syncthreads();
-    trsv_upper_block(1, (int)n, R, K.ldr, xs, scratch);
+    trsv_upper_block(1, (int)n, R, K.ldr, xs, scratch, ws + L.dinv);
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
       const double t = (coef != 0.0) ? xs[i] + coef * v_in[i] : xs[i];
       v_out[i] = t;
@@ -344,13 +345,14 @@ __device__ inline double apply_adjoint(const LsqrConst& K, const double* __restr
   return block_sum(ss, red);
 }
 
-// out = R^{-1} v (or v); v complete before the call, out complete after
+// out = R^{-1} v (or v); v complete before the call, out complete after;
+// dinv: the inverses of R's 8 x 8 diagonal sub-blocks (trsv_dinv8)
 __device__ inline void apply_forward(const LsqrConst& K, const double* __restrict__ R, const double* v,
-                                     double* out, double* xs, double* scratch) {
+                                     double* out, double* xs, double* scratch, const double* dinv) {
   if (K.has_r) {
     for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) xs[i] = v[i];
     __syncthreads();
-    trsv_upper_block(0, (int)K.n, R, K.ldr, xs, scratch);
+    trsv_upper_block(0, (int)K.n, R, K.ldr, xs, scratch, dinv);
     for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) out[i] = xs[i];
   } else {
     for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) out[i] = v[i];
@@ -374,10 +376,11 @@ __global__ __launch_bounds__(1024) void lsqr_init_kernel(LsqrConst K, const doub
     x[i] = K.has_x0 ? x0[i] : 0.0;
     if (var) var[i] = 0.0;
   }
+  if (K.has_r) trsv_dinv8((int)K.n, R, K.ldr, ws + L.dinv);  // once per solve: R is fixed
   __syncthreads();
   if (K.has_x0) {
     double* xs = K.xs_global ? ws + L.xs : smem;
-    apply_forward(K, R, x, ws + L.work, xs, K.xs_global ? smem : smem + K.n);
+    apply_forward(K, R, x, ws + L.work, xs, K.xs_global ? smem : smem + K.n, ws + L.dinv);
   } else {
     for (int64_t i = threadIdx.x; i < K.n; i += blockDim.x) ws[L.work + i] = 0.0;
   }
@@ -429,7 +432,7 @@ __global__ __launch_bounds__(1024) void lsqr_tail_kernel(LsqrConst K, const doub
     __syncthreads();
     const double arnorm = alfa * beta;
     const bool stop = (arnorm == 0.0) || (K.iter_lim <= 0);
-    if (!stop) apply_forward(K, R, v, ws + L.work, xs, scratch);
+    if (!stop) apply_forward(K, R, v, ws + L.work, xs, scratch, ws + L.dinv);
     __syncthreads();
     if (threadIdx.x == 0) {
       S->bnorm = bnorm;
@@ -551,7 +554,7 @@ __global__ __launch_bounds__(1024) void lsqr_tail_kernel(LsqrConst K, const doub
     bc[0] = (istop != 0) ? 1.0 : 0.0;
   }
   __syncthreads();
-  if (bc[0] == 0.0) apply_forward(K, R, v, ws + L.work, xs, scratch);
+  if (bc[0] == 0.0) apply_forward(K, R, v, ws + L.work, xs, scratch, ws + L.dinv);
   if (threadIdx.x == 0 && bc[0] != 0.0) S->done = 1;
 }
 
@@ -562,7 +565,7 @@ __global__ __launch_bounds__(1024) void lsqr_xorig_kernel(LsqrConst K, const dou
   extern __shared__ double smem[];
   const LsqrLayout L = lsqr_layout(K.m, K.n);
   double* xs = K.xs_global ? ws + L.xs : smem;
-  apply_forward(K, R, x, xo, xs, K.xs_global ? smem : smem + K.n);
+  apply_forward(K, R, x, xo, xs, K.xs_global ? smem : smem + K.n, ws + L.dinv);
 }
 
 int launch_pass(const LsqrConst& K, const double* A, double* ws, hipStream_t st) {

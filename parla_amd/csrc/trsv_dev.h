@@ -30,16 +30,47 @@ __device__ __forceinline__ void trsv_load_diag(const double* __restrict__ R, int
   }
 }
 
+// dinv[q] (8 x 8, row-major) = the inverse of R's 8 x 8 diagonal sub-block
+// at rows / columns 8 q .. (identity where it passes n), for q < ceil(n / 8):
+// one thread per (sub-block, column) by back substitution.  All threads of
+// the block call this; the caller synchronises before dinv is read.
+__device__ inline void trsv_dinv8(int n, const double* __restrict__ R, int64_t ldr, double* __restrict__ dinv) {
+  const int nq = (n + 7) / 8;
+  for (int t = threadIdx.x; t < nq * 8; t += blockDim.x) {
+    const int q = t >> 3, j = t & 7, b0 = 8 * q;
+    double z[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) z[i] = (i == j) ? 1.0 : 0.0;
+    if (b0 + j < n) {  // a column of the triangle: every row i <= j is inside it
+#pragma unroll
+      for (int i = 7; i >= 0; --i) {
+        if (i > j) continue;
+        const double* row = R + (int64_t)(b0 + i) * ldr + b0;
+        double s = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = i + 1; k < 8; ++k)
+          if (k <= j) s -= row[k] * z[k];
+        z[i] = s / row[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dinv[(int64_t)q * 64 + i * 8 + j] = z[i];
+  }
+}
+
 // All threads of the block (blockDim.x a multiple of 64, at most 1024) must
 // call this.  trans = 0 solves R x = b, trans = 1 solves R^T x = b; xs holds b
 // on entry and x on exit, in LDS or (for n beyond what LDS holds) in global
 // memory that only this workgroup touches.  scratch: TRSV_SCRATCH_DOUBLES
 // doubles of LDS.  DPT: diagonal-block elements a thread keeps in registers;
 // 4096 / blockDim.x covers the whole block, a smaller value is still correct
-// (the rest is read from R when it is stored to LDS).
+// (the rest is read from R when it is stored to LDS).  dinv (optional, from
+// trsv_dinv8): the 64 x 64 diagonal blocks are then solved in eight steps of
+// 8 rows, each a product with the sub-block's inverse, instead of 64 steps of
+// one row; the inverses' application error is of order u kappa(8 x 8 block).
 template <int DPT = 4>
 __device__ inline void trsv_upper_block(int trans, int n, const double* __restrict__ R, int64_t ldr,
-                                        double* xs, double* scratch) {
+                                        double* xs, double* scratch, const double* __restrict__ dinv = nullptr) {
   double(*Rd)[65] = reinterpret_cast<double(*)[65]>(scratch);
   double(*red)[64] = reinterpret_cast<double(*)[64]>(scratch + 64 * 65);
   double* rdiag = scratch + 64 * 65 + 16 * 64;  // 1 / R_ii of the current block
@@ -123,7 +154,46 @@ __device__ inline void trsv_upper_block(int trans, int n, const double* __restri
       const int nblk = trans ? bi + 1 : (nb - 2 - bi);
       trsv_load_diag(R, ldr, nblk * 64, min(64, n - nblk * 64), dnext);
     }
-    if (wave == 0) {
+    if (wave == 0 && dinv) {
+      double y = (lane < bs) ? xs[r0 + lane] : 0.0;
+      if (trans) {
+        double t = 0.0;
+        for (int w = 0; w < nwaves; ++w) t += red[w][lane];
+        y -= t;
+      }
+      // this lane's row (trans: column) of its sub-block's inverse (zero past
+      // the triangle, whose sub-blocks dinv does not hold)
+      const double* dsb = dinv + (int64_t)((r0 + (lane & ~7)) >> 3) * 64;
+      double di[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        di[k] = (lane < bs) ? (trans ? dsb[k * 8 + (lane & 7)] : dsb[(lane & 7) * 8 + k]) : 0.0;
+      const int nsb = (bs + 7) / 8;
+      for (int q = 0; q < nsb; ++q) {
+        const int i0 = 8 * (trans ? q : (nsb - 1 - q));
+        double r[8], yk[8], xk[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r[k] = trans ? Rd[i0 + k][lane] : Rd[lane][i0 + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) yk[k] = wave_bcast(y, i0 + k);
+        // both 8-term sums as four pairs added in a tree: a dependent chain of
+        // four operations instead of eight
+        double px[4], pu[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) px[k] = fma(di[2 * k + 1], yk[2 * k + 1], di[2 * k] * yk[2 * k]);
+        const double xl = (px[0] + px[1]) + (px[2] + px[3]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xk[k] = wave_bcast(xl, i0 + k);
+        // every lane forms the update (so R's LDS reads stay at the top of the
+        // step, ahead of the broadcasts), the lanes it applies to keep it
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pu[k] = fma(r[2 * k + 1], xk[2 * k + 1], r[2 * k] * xk[2 * k]);
+        const double yu = y - ((pu[0] + pu[1]) + (pu[2] + pu[3]));
+        const bool below = trans ? (lane >= i0 + 8) : (lane < i0);
+        y = (lane >= i0 && lane < i0 + 8) ? xl : (below ? yu : y);
+      }
+      if (lane < bs) xs[r0 + lane] = y;
+    } else if (wave == 0) {
       double y = (lane < bs) ? xs[r0 + lane] : 0.0;
       const double rd = rdiag[lane];  // lane i holds 1 / R_ii
       if (trans) {
