@@ -451,6 +451,18 @@ int pamd_trsm_right_upper_rec_f64(int64_t m, int64_t n, const double* R,
                                   void* workspace, int64_t workspace_bytes,
                                   void* stream);
 
+/* X = A R^{-1} out of place (A is not modified) with the FP32 rounding of X
+ * in X32 (m x n, leading dimension ld32): the same recursion and results as
+ * pamd_trsm_right_upper_rec_f64 on a copy of A, without the copy of A and
+ * the separate conversion pass.  Forms the FP32-stored preconditioned
+ * operator of the mixed-precision solver (comps/preconditioning.py:16-41
+ * applies A R^{-1} implicitly); workspace as for pamd_gemm_f64 with M = m. */
+int pamd_trsm_right_upper_rec_f32out(int64_t m, int64_t n, const double* R,
+                                     int64_t ldr, const double* A, int64_t lda,
+                                     double* X, int64_t ldx, float* X32,
+                                     int64_t ld32, void* workspace,
+                                     int64_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Learned leverage-score sketch                                             */
 /* ------------------------------------------------------------------------ */

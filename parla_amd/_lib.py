@@ -115,6 +115,7 @@ SIGNATURES = {
     "pamd_bt_dot": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "pamd_nrm2_large_f64": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp]),
     "pamd_trsm_right_upper_rec_f64": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
+    "pamd_trsm_right_upper_rec_f32out": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
     "pamd_mlp_nparams": (c_int, []),
     "pamd_mlp_fwd": (c_int, [c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "pamd_mlp_bwd_workspace": (c_int, [c_i64, ctypes.POINTER(c_i64)]),

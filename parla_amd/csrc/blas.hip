@@ -100,12 +100,13 @@ __device__ __forceinline__ void store_b(double (*Bs)[BN + 1], const double (&r)[
 // 2 x 2, each owning a (BM/2) x (BN/2) quadrant of 16 x 16 MFMA blocks.
 // grid: (ceil(N/BN), ceil(M/BM), splits).  Slice z covers K range
 // [z*kchunk, min(K, (z+1)*kchunk)).  With splits > 1 the raw product of each
-// slice goes to part[z] (M x N, ld N); otherwise C = alpha*prod + beta*C.
+// slice goes to part[z] (M x N, ld N); otherwise C = alpha*prod + beta*Cin
+// (Cin, ldcin: the input C, which may be C itself).
 template <int TA, int TB, int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk,
                                                        double alpha, const double* __restrict__ A, int64_t lda,
                                                        const double* __restrict__ B, int64_t ldb, double beta,
-                                                       double* __restrict__ C, int64_t ldc,
+                                                       const double* Cin, int64_t ldcin, double* C, int64_t ldc,
                                                        double* __restrict__ part) {
   constexpr int IM = BM / 32, JN = BN / 32;  // MFMA blocks per wavefront
   __shared__ double As[GBK][BM + 1];
@@ -185,8 +186,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int64_t M, int64_t N, int
           if (part) {
             part[((int64_t)tz * M + row) * N + col] = v;
           } else {
-            double* c = C + row * ldc + col;
-            *c = (beta == 0.0) ? alpha * v : alpha * v + beta * (*c);
+            C[row * ldc + col] = (beta == 0.0) ? alpha * v : alpha * v + beta * Cin[row * ldcin + col];
           }
         }
       }
@@ -199,16 +199,15 @@ inline int gemm_edge(int64_t d) {
   return (p96 * 20 <= p64 * 19) ? 96 : 64;
 }
 
-// C = alpha * sum_z part[z] + beta * C, summed in slice order.
+// C = alpha * sum_z part[z] + beta * Cin, summed in slice order.
 __global__ void splitk_reduce_kernel(int64_t M, int64_t N, int splits, double alpha, const double* __restrict__ part,
-                                     double beta, double* __restrict__ C, int64_t ldc) {
+                                     double beta, const double* Cin, int64_t ldcin, double* C, int64_t ldc) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= M * N) return;
   const int64_t row = idx / N, col = idx % N;
   double s = 0.0;
   for (int z = 0; z < splits; ++z) s += part[z * M * N + idx];
-  double* c = C + row * ldc + col;
-  *c = (beta == 0.0) ? alpha * s : alpha * s + beta * (*c);
+  C[row * ldc + col] = (beta == 0.0) ? alpha * s : alpha * s + beta * Cin[row * ldcin + col];
 }
 
 int gemm_splits(int64_t M, int64_t N, int64_t K) {
@@ -360,6 +359,15 @@ extern "C" int pamd_gemm_workspace(int64_t M, int64_t N, int64_t K, int64_t* byt
 extern "C" int pamd_gemm_f64(int transA, int transB, int64_t M, int64_t N, int64_t K, double alpha,
                              const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
                              int64_t ldc, void* workspace, int64_t workspace_bytes, void* stream) {
+  return pamd_gemm_cin_f64(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, C, ldc, workspace,
+                           workspace_bytes, stream);
+}
+
+// C = alpha op(A) op(B) + beta Cin with the input C read from Cin (which may
+// be C): the library's own out-of-place updates (recursive TRSM) use it.
+int pamd_gemm_cin_f64(int transA, int transB, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+                      int64_t lda, const double* B, int64_t ldb, double beta, const double* Cin, int64_t ldcin,
+                      double* C, int64_t ldc, void* workspace, int64_t workspace_bytes, void* stream) {
   PAMD_ARG_CHECK(transA == 0 || transA == 1, 1);
   PAMD_ARG_CHECK(transB == 0 || transB == 1, 2);
   PAMD_ARG_CHECK(M >= 0, 3);
@@ -371,6 +379,8 @@ extern "C" int pamd_gemm_f64(int transA, int transB, int64_t M, int64_t N, int64
   PAMD_ARG_CHECK(ldb >= (transB ? K : N) && ldb >= 1, 10);
   PAMD_ARG_CHECK(C != nullptr || M * N == 0, 12);
   PAMD_ARG_CHECK(ldc >= N && ldc >= 1, 13);
+  PAMD_ARG_CHECK(Cin != nullptr || M * N == 0 || beta == 0.0, 12);
+  PAMD_ARG_CHECK(ldcin >= N && ldcin >= 1, 13);
   if (M == 0 || N == 0) return PAMD_OK;
   hipStream_t st = (hipStream_t)stream;
   int splits = gemm_splits(M, N, K);
@@ -382,7 +392,8 @@ extern "C" int pamd_gemm_f64(int transA, int transB, int64_t M, int64_t N, int64
   const int bm = gemm_edge(M), bn = gemm_edge(N);
   dim3 grid((unsigned)pamd_cdiv(N, bn), (unsigned)pamd_cdiv(M, bm), (unsigned)splits);
 #define PAMD_GEMM(TA, TB, BM, BN) \
-  gemm_f64_kernel<TA, TB, BM, BN><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, part)
+  gemm_f64_kernel<TA, TB, BM, BN><<<grid, 256, 0, st>>>(M, N, K, kchunk, alpha, A, lda, B, ldb, beta, Cin, ldcin, \
+                                                        C, ldc, part)
 #define PAMD_GEMM_T(BM, BN)                                  \
   do {                                                       \
     if (transA == 0 && transB == 0) PAMD_GEMM(0, 0, BM, BN); \
@@ -399,7 +410,8 @@ extern "C" int pamd_gemm_f64(int transA, int transB, int64_t M, int64_t N, int64
   PAMD_LAUNCH_CHECK();
   if (splits > 1) {
     const int64_t tot = M * N;
-    splitk_reduce_kernel<<<(unsigned)pamd_cdiv(tot, 256), 256, 0, st>>>(M, N, splits, alpha, part, beta, C, ldc);
+    splitk_reduce_kernel<<<(unsigned)pamd_cdiv(tot, 256), 256, 0, st>>>(M, N, splits, alpha, part, beta, Cin, ldcin,
+                                                                        C, ldc);
     PAMD_LAUNCH_CHECK();
   }
   return PAMD_OK;

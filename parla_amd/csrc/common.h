@@ -42,6 +42,12 @@
     if (e_ != hipSuccess) return PAMD_ERR_HIP;          \
   } while (0)
 
+// Library-internal entry shared between sources (not part of the C ABI):
+// C = alpha op(A) op(B) + beta Cin, the input C read from Cin (blas.hip).
+int pamd_gemm_cin_f64(int transA, int transB, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+                      int64_t lda, const double* B, int64_t ldb, double beta, const double* Cin, int64_t ldcin,
+                      double* C, int64_t ldc, void* workspace, int64_t workspace_bytes, void* stream);
+
 __host__ __device__ static inline int64_t pamd_cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 __host__ __device__ static inline int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
 __host__ __device__ static inline int64_t imax64(int64_t a, int64_t b) { return a > b ? a : b; }

@@ -653,16 +653,21 @@ __global__ __launch_bounds__(64 * RED_WAVES) void residual_reduce_kernel(int64_t
 }
 
 // ---------------------------------------------------------------------------
-// TRSM leaf: X (m x nb, nb <= LEAF) <- X R^{-1}, R upper nb x nb; thread per row
+// TRSM leaf: X (m x nb, nb <= LEAF) = Xin R^{-1}, R upper nb x nb; thread per
+// row; with X32, the FP32 rounding of X is written too.  Xin may be X: each
+// thread reads its whole row before writing it and touches no other row, so
+// the pointers are restrict-qualified either way.
 // ---------------------------------------------------------------------------
 constexpr int LEAF = 32;
 
 // FULL: nb == LEAF, rows moved straight between memory and registers; else the
 // partial last leaf goes through an LDS copy (a register array may only be
 // indexed with compile-time indices, and nb is a run-time value)
-template <bool FULL>
+template <bool FULL, bool W32>
 __global__ __launch_bounds__(256) void trsm_leaf_kernel(int64_t m, int nb, const double* __restrict__ R, int64_t ldr,
-                                                        double* __restrict__ X, int64_t ldx) {
+                                                        const double* __restrict__ Xin, int64_t ldxin,
+                                                        double* __restrict__ X, int64_t ldx, float* __restrict__ X32,
+                                                        int64_t ld32) {
   __shared__ double Rl[LEAF][LEAF + 1];
   __shared__ double rinv[LEAF];
   __shared__ double xs[FULL ? 1 : 256][LEAF + 1];
@@ -675,13 +680,14 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int64_t m, int nb, const
   __syncthreads();
   const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= m) return;
+  const double* xi = Xin + row * ldxin;
   double* xr = X + row * ldx;
   double x[LEAF];
   if constexpr (FULL) {
 #pragma unroll
-    for (int j = 0; j < LEAF; ++j) x[j] = xr[j];
+    for (int j = 0; j < LEAF; ++j) x[j] = xi[j];
   } else {
-    for (int j = 0; j < LEAF; ++j) xs[threadIdx.x][j] = (j < nb) ? xr[j] : 0.0;
+    for (int j = 0; j < LEAF; ++j) xs[threadIdx.x][j] = (j < nb) ? xi[j] : 0.0;
 #pragma unroll
     for (int j = 0; j < LEAF; ++j) x[j] = xs[threadIdx.x][j];
   }
@@ -698,33 +704,53 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int64_t m, int nb, const
     for (int i = 0; i < LEAF; ++i)
       if (i > j) x[i] -= rowj[i] * y;
   }
+  // W32 a template argument: a run-time test here makes the compiler hoist the
+  // whole block's R reads into registers and spill
   if constexpr (FULL) {
 #pragma unroll
     for (int j = 0; j < LEAF; ++j) xr[j] = x[j];
+    if constexpr (W32) {
+#pragma unroll
+      for (int j = 0; j < LEAF; ++j) X32[row * ld32 + j] = (float)x[j];
+    }
   } else {
 #pragma unroll
     for (int j = 0; j < LEAF; ++j) xs[threadIdx.x][j] = x[j];
     for (int j = 0; j < nb; ++j) xr[j] = xs[threadIdx.x][j];
+    if constexpr (W32)
+      for (int j = 0; j < nb; ++j) X32[row * ld32 + j] = (float)xs[threadIdx.x][j];
   }
 }
 
-int trsm_rec(int64_t m, int64_t n, const double* R, int64_t ldr, double* X, int64_t ldx, void* ws, int64_t wsb,
-             hipStream_t st) {
+// X = Xin R^{-1} (Xin may be X), recursively over column halves: X1 = Xin1
+// R11^{-1}, X2 = (Xin2 - X1 R12) R22^{-1}.  Only the leftmost leaf of each
+// level and the updates read Xin, so an out-of-place solve costs no copy of
+// Xin.  With X32, every leaf also writes its columns' FP32 rounding.
+int trsm_rec(int64_t m, int64_t n, const double* R, int64_t ldr, const double* Xin, int64_t ldxin, double* X,
+             int64_t ldx, float* X32, int64_t ld32, void* ws, int64_t wsb, hipStream_t st) {
   if (n <= LEAF) {
-    if (n == LEAF)
-      trsm_leaf_kernel<true><<<(unsigned)pamd_cdiv(m, 256), 256, 0, st>>>(m, (int)n, R, ldr, X, ldx);
+    const unsigned g = (unsigned)pamd_cdiv(m, 256);
+    const int nb = (int)n;
+    if (n == LEAF && X32)
+      trsm_leaf_kernel<true, true><<<g, 256, 0, st>>>(m, nb, R, ldr, Xin, ldxin, X, ldx, X32, ld32);
+    else if (n == LEAF)
+      trsm_leaf_kernel<true, false><<<g, 256, 0, st>>>(m, nb, R, ldr, Xin, ldxin, X, ldx, X32, ld32);
+    else if (X32)
+      trsm_leaf_kernel<false, true><<<g, 256, 0, st>>>(m, nb, R, ldr, Xin, ldxin, X, ldx, X32, ld32);
     else
-      trsm_leaf_kernel<false><<<(unsigned)pamd_cdiv(m, 256), 256, 0, st>>>(m, (int)n, R, ldr, X, ldx);
+      trsm_leaf_kernel<false, false><<<g, 256, 0, st>>>(m, nb, R, ldr, Xin, ldxin, X, ldx, X32, ld32);
     PAMD_LAUNCH_CHECK();
     return PAMD_OK;
   }
   const int64_t n1 = ((n / 2 + LEAF - 1) / LEAF) * LEAF;
-  int rc = trsm_rec(m, n1, R, ldr, X, ldx, ws, wsb, st);
+  int rc = trsm_rec(m, n1, R, ldr, Xin, ldxin, X, ldx, X32, ld32, ws, wsb, st);
   if (rc) return rc;
-  // X2 -= X1 R12
-  rc = pamd_gemm_f64(0, 0, m, n - n1, n1, -1.0, X, ldx, R + n1, ldr, 1.0, X + n1, ldx, ws, wsb, st);
+  // X2 = Xin2 - X1 R12
+  rc = pamd_gemm_cin_f64(0, 0, m, n - n1, n1, -1.0, X, ldx, R + n1, ldr, 1.0, Xin + n1, ldxin, X + n1, ldx, ws, wsb,
+                         st);
   if (rc) return rc;
-  return trsm_rec(m, n - n1, R + n1 * ldr + n1, ldr, X + n1, ldx, ws, wsb, st);
+  return trsm_rec(m, n - n1, R + n1 * ldr + n1, ldr, X + n1, ldx, X + n1, ldx, X32 ? X32 + n1 : nullptr, ld32, ws,
+                  wsb, st);
 }
 
 inline int64_t np_grid(int64_t m) { return imax64(1, imin64(NP_MAX_GRID, pamd_cdiv(m, 64))); }
@@ -912,5 +938,22 @@ extern "C" int pamd_trsm_right_upper_rec_f64(int64_t m, int64_t n, const double*
   PAMD_ARG_CHECK(X != nullptr || m * n == 0, 5);
   PAMD_ARG_CHECK(ldx >= n && ldx >= 1, 6);
   if (m == 0 || n == 0) return PAMD_OK;
-  return trsm_rec(m, n, R, ldr, X, ldx, workspace, workspace_bytes, (hipStream_t)stream);
+  return trsm_rec(m, n, R, ldr, X, ldx, X, ldx, nullptr, 0, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+extern "C" int pamd_trsm_right_upper_rec_f32out(int64_t m, int64_t n, const double* R, int64_t ldr, const double* A,
+                                                int64_t lda, double* X, int64_t ldx, float* X32, int64_t ld32,
+                                                void* workspace, int64_t workspace_bytes, void* stream) {
+  PAMD_ARG_CHECK(m >= 0, 1);
+  PAMD_ARG_CHECK(n >= 0, 2);
+  PAMD_ARG_CHECK(R != nullptr || n == 0, 3);
+  PAMD_ARG_CHECK(ldr >= n && ldr >= 1, 4);
+  PAMD_ARG_CHECK(A != nullptr || m * n == 0, 5);
+  PAMD_ARG_CHECK(lda >= n && lda >= 1, 6);
+  PAMD_ARG_CHECK(X != nullptr || m * n == 0, 7);
+  PAMD_ARG_CHECK(ldx >= n && ldx >= 1, 8);
+  PAMD_ARG_CHECK(X32 != nullptr || m * n == 0, 9);
+  PAMD_ARG_CHECK(ld32 >= n && ld32 >= 1, 10);
+  if (m == 0 || n == 0) return PAMD_OK;
+  return trsm_rec(m, n, R, ldr, A, lda, X, ldx, X32, ld32, workspace, workspace_bytes, (hipStream_t)stream);
 }

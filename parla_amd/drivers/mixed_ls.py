@@ -110,10 +110,7 @@ class SPOMixed(OverLstsqSolver):
         fac = ops.qr_r_(Mx, n)
         R = torch.triu(fac.R_view).contiguous()
         z = Mx[:n, n].clone()
-        B = At.clone()
-        ops.trsm_rec_(B, R)
-        B32 = B.to(torch.float32)
-        del B
+        B32 = ops.trsm_rec_f32(At, R)
         log.time_factor = clock() - tic
 
         tic = clock()

@@ -816,6 +816,25 @@ class Residual:
         return g, rss, r
 
 
+def trsm_rec_f32(A, R):
+    """FP32 rounding of A R^{-1} (R upper), A unmodified: the recursive TRSM
+    out of place with the FP32 copy written by its leaves (no copy of A, no
+    separate conversion pass).  Equal bit for bit to trsm_rec_(A.clone(), R)
+    followed by .to(torch.float32)."""
+    m, n = A.shape
+    if A.stride(1) != 1 and n > 1:
+        A = A.contiguous()
+    X = torch.empty((m, n), dtype=F64, device=A.device)
+    X32 = torch.empty((m, n), dtype=torch.float32, device=A.device)
+    ws_bytes = _query(lib().pamd_gemm_workspace, m, n, n)
+    ws = workspace(ws_bytes, A.device)
+    check(lib().pamd_trsm_right_upper_rec_f32out(m, n, _p(R), max(R.stride(0), n, 1), _p(A), max(A.stride(0), n, 1),
+                                                 _p(X), max(n, 1), _p(X32), max(n, 1), _p(ws), ws_bytes,
+                                                 stream(A.device)), "pamd_trsm_right_upper_rec_f32out")
+    del X
+    return X32
+
+
 def trsm_rec_(X, R):
     """In place X <- X R^{-1} (R upper), recursive MFMA TRSM for large m."""
     m, n = X.shape

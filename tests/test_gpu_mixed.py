@@ -122,6 +122,13 @@ def test_trsm_recursive(ops, m, n):
     import scipy.linalg as sla
     ref = sla.solve_triangular(R, A.T, trans='T').T
     np.testing.assert_allclose(X, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    # out of place with the FP32 copy: A untouched, and bit for bit the FP32
+    # rounding of the in-place result (same recursion, same sums)
+    Ad = _dev(A)
+    X32 = ops.trsm_rec_f32(Ad, _dev(R)).cpu().numpy()
+    assert np.array_equal(Ad.cpu().numpy(), A)
+    assert X32.dtype == np.float32
+    assert np.array_equal(X32, X.astype(np.float32))
 
 
 @pytest.mark.parametrize("cond", [1e4, 1e8])
