@@ -513,6 +513,12 @@ def svd_small(M):
 
 JACOBI_MAX = 1024          # largest dimension the LDS-resident Jacobi pair handles
 JACOBI_SWEEPS = 40
+# Sweeps enqueued first.  Every sweep is nb - 1 launches, and sweeps after
+# convergence still cost a launch each (their rounds return at once): the
+# preconditioned matrices of the drivers (R^T of a QR) converge in 8-10
+# sweeps, so 12 are enqueued, and only a matrix that needs more is solved
+# again from the start with JACOBI_SWEEPS.
+JACOBI_SWEEPS_FIRST = 12
 
 
 def svd_jacobi(X):
@@ -528,11 +534,13 @@ def svd_jacobi(X):
     info = torch.zeros(1, dtype=torch.int32, device=dev)
     ws = workspace(_query(lib().pamd_svd_jacobi_workspace, m, n), dev)
     with _Timed("svd_jacobi", 0.0, 8.0 * (m * n + n * n)):
-        check(lib().pamd_svd_jacobi_f64(m, n, _p(Xc), max(Xc.stride(0), n, 1), _p(U), n, _p(s), _p(Vt), n,
-                                        JACOBI_SWEEPS, _p(ws), _p(info), stream(dev)), "pamd_svd_jacobi_f64")
-    if int(info.item()) != 0:
-        raise np.linalg.LinAlgError(f"Jacobi SVD did not converge in {JACOBI_SWEEPS} sweeps")
-    return U, s, Vt
+        for sweeps in (JACOBI_SWEEPS_FIRST, JACOBI_SWEEPS):
+            # the entry point writes info (0: a sweep within the limit rotated nothing)
+            check(lib().pamd_svd_jacobi_f64(m, n, _p(Xc), max(Xc.stride(0), n, 1), _p(U), n, _p(s), _p(Vt), n,
+                                            sweeps, _p(ws), _p(info), stream(dev)), "pamd_svd_jacobi_f64")
+            if int(info.item()) == 0:
+                return U, s, Vt
+    raise np.linalg.LinAlgError(f"Jacobi SVD did not converge in {JACOBI_SWEEPS} sweeps")
 
 
 def thin_svd(X):
