@@ -9,7 +9,9 @@ as the target backward error eta(x) = ||A^T r|| / (||A||_F ||r||).
      pieces per 256-row chunk, and [S A | S b] is formed on BF16 MFMA with
      exact FP32 partial sums and FP64 accumulation: it equals S times A
      truncated to 9 * npieces bits below each chunk-column maximum.
-  2. Factor.  Householder QR of S A in FP64; z = Q^T S b.
+  2. Factor.  QR of S A in FP64; z = Q^T S b.  Sketches of at most 1024 rows
+     by CholeskyQR2 (Householder when S A is too ill-conditioned for it),
+     taller ones by the communication-avoiding Householder QR.
   3. Precondition explicitly.  B = A R^{-1} in FP64 by a recursive MFMA TRSM,
      stored in FP32 (kappa(B) is about 3, so FP32 storage costs nothing in
      the refinement's contraction rate).
@@ -107,9 +109,15 @@ class SPOMixed(OverLstsqSolver):
         log.time_sketch = clock() - tic
 
         tic = clock()
-        fac = ops.qr_r_(Mx, n)
-        R = torch.triu(fac.R_view).contiguous()
-        z = Mx[:n, n].clone()
+        # sketches below the communication-avoiding QR's range: CholeskyQR2
+        # of S A when S A is well enough conditioned for it, else Householder
+        rz = ops.sketch_rz_cholqr2(Mx, n) if d < ops.QR_R_MIN_ROWS else None
+        if rz is not None:
+            R, z = rz
+        else:
+            fac = ops.qr_r_(Mx, n)
+            R = torch.triu(fac.R_view).contiguous()
+            z = Mx[:n, n].clone()
         B32 = ops.trsm_rec_f32(At, R)
         log.time_factor = clock() - tic
 

@@ -447,6 +447,37 @@ def cholqr2(Y, rounds=2):
     return Q
 
 
+def sketch_rz_cholqr2(M, n):
+    """(R, z) of a sketch M = [S A | S b] (d x (n + 1)) by CholeskyQR2 on
+    S A: R1 = chol((S A)^T S A), Q1 = S A R1^{-1}, R2 = chol(Q1^T Q1),
+    Q = Q1 R2^{-1}; R = R2 R1 (positive diagonal) and z = Q^T S b, the same
+    R and Q^T S b as a Householder QR up to the signs of R's rows, to
+    O(u kappa) for kappa(S A) << u^{-1/2}.  None when S A is too
+    ill-conditioned for it (a pivot fails or R1's diagonal spread exceeds
+    CHOLQR_MAX_COND); the caller then factors M by Householder.  Two Gram
+    GEMMs, two Cholesky factorisations and two MFMA triangular solves: on
+    sketches of a few hundred rows this is much shorter than a panel-by-panel
+    Householder QR."""
+    d, ncols = M.shape
+    if n < 1 or n > CHOLQR_MAX_K or d < n or ncols != n + 1:
+        return None
+    SA = M[:, :n]
+    info = torch.zeros(2, dtype=torch.int32, device=M.device)
+    G1, Q1 = _chol_round(SA, info[0:1])
+    G2, Q = _chol_round(Q1, info[1:2])
+    stats = torch.empty(4, dtype=F64, device=M.device)
+    check(lib().pamd_cholqr_stats_f64(n, _p(G1), G1.stride(0), _p(info), None, _p(stats), stream(M.device)),
+          "pamd_cholqr_stats_f64")
+    dd = stats.cpu().numpy()
+    if dd[3] != 0 or not np.all(np.isfinite(dd)) or dd[0] <= 0 or dd[1] / dd[0] > CHOLQR_MAX_COND:
+        return None
+    if int(info[1].item()) != 0:
+        return None
+    R = gemm(torch.triu(G2), torch.triu(G1))
+    z = gemm(Q.t(), M[:, n:]).reshape(n)
+    return R, z
+
+
 def orth_q(Y, rounds=2):
     """Orthonormal basis of range(Y) (m x min(m, n)).  CholeskyQR2 on MFMA when Y
     is tall and well conditioned, else Householder; the two agree up to the sign

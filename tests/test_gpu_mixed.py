@@ -131,6 +131,50 @@ def test_trsm_recursive(ops, m, n):
     assert np.array_equal(X32, X.astype(np.float32))
 
 
+def _sketch_like(d, n, cond, seed):
+    """[S A | S b]: Haar factors with geometric singular values, b mostly in range."""
+    import scipy.linalg as sla
+    rng = np.random.default_rng(seed)
+    U = sla.qr(rng.standard_normal((d, n)), mode="economic")[0]
+    V = sla.qr(rng.standard_normal((n, n)))[0]
+    SA = (U * np.geomspace(1.0, 1.0 / cond, n)) @ V.T
+    Sb = SA @ rng.standard_normal(n) + 1e-3 * rng.standard_normal(d)
+    return np.column_stack([SA, Sb])
+
+
+@pytest.mark.parametrize("d,n,cond", [(800, 200, 1e6), (480, 120, 1e4), (300, 257, 1e2), (64, 1, 1.0)])
+def test_sketch_rz_cholqr2_matches_householder(ops, d, n, cond):
+    """R and z = Q^T S b from CholeskyQR2 against LAPACK's Householder QR of the
+    same sketch, after making R's diagonal positive: both are backward stable,
+    so they agree to a small multiple of u kappa(S A)."""
+    import scipy.linalg as sla
+    _say(f"sketch R, z by CholeskyQR2 d={d} n={n} cond={cond:g}")
+    M = _sketch_like(d, n, cond, d + n)
+    rz = ops.sketch_rz_cholqr2(_dev(M), n)
+    assert rz is not None
+    R, z = (t.cpu().numpy() for t in rz)
+    Ql, Rl = sla.qr(M[:, :n], mode="economic")
+    sg = np.sign(np.diag(Rl))
+    Rl, zl = sg[:, None] * Rl, sg * (Ql.T @ M[:, n])
+    assert np.all(np.diag(R) > 0) and np.all(np.tril(R, -1) == 0)
+    tol = 100 * np.finfo(float).eps * cond * np.sqrt(n)
+    assert np.linalg.norm(R - Rl) <= tol * np.linalg.norm(Rl)
+    assert np.linalg.norm(z - zl) <= tol * np.linalg.norm(M[:, n])
+    # the preconditioned sketch S A R^{-1} is as orthonormal as with LAPACK's R
+    # (what the refinement's operator B = A R^{-1} relies on); both measured by
+    # the same host solve, whose own error is of order u kappa
+    def orth_err(Rx):
+        Qc = sla.solve_triangular(Rx, M[:, :n].T, trans="T").T
+        return np.linalg.norm(Qc.T @ Qc - np.eye(n))
+    assert orth_err(R) <= 10 * orth_err(Rl) + 1e-13 * np.sqrt(n)
+
+
+def test_sketch_rz_cholqr2_declines_ill_conditioned(ops):
+    """kappa(S A) = 1e10 is beyond CholeskyQR2: None, and SPOMixed factors by Householder."""
+    _say("sketch R, z by CholeskyQR2 declines kappa 1e10")
+    assert ops.sketch_rz_cholqr2(_dev(_sketch_like(600, 100, 1e10, 5)), 100) is None
+
+
 @pytest.mark.parametrize("cond", [1e4, 1e8])
 def test_spo_mixed_reaches_fp64_floor(ops, cond):
     """The refined solution's backward error (exactly evaluated) is below
