@@ -39,8 +39,11 @@ from parla_amd.drivers.least_squares import OverLstsqSolver, dim_checks
 def cg_normal(op, rhs, max_iter, tol, check_every=4):
     """z ~= (B^T B)^{-1} rhs by conjugate gradients.  The recurrences (dots,
     alpha, beta, the z / r / p updates) run in one device kernel per step
-    (pamd_bt_cg on a one-problem descriptor); the host reads ||r|| once every
-    `check_every` steps."""
+    (pamd_bt_cg on a one-problem descriptor).  The host reads ||r|| after
+    `check_every` steps; from the drop measured there it predicts the step at
+    which ||r|| <= tol ||r_0|| (CG on a well-conditioned B contracts at a
+    near-constant rate) and reads again there, then every `check_every`
+    steps.  Each read is a synchronisation, so fewer reads keep the GPU fed."""
     n = rhs.numel()
     dev = rhs.device
     desc = torch.tensor([[1, n, 0, 0, 0, 0, 0, 0]], dtype=torch.int64, device=dev)
@@ -53,14 +56,22 @@ def cg_normal(op, rhs, max_iter, tol, check_every=4):
                                ops._p(z), st), "pamd_bt_cg")
 
     step(0)
-    rr0 = state[0].clone()
-    it = 0
+    rr0 = state[0:1].clone()
+    it, check_at, first = 0, check_every, True
     while it < max_iter:
         op(p, q)
         step(1)
         it += 1
-        if it % check_every == 0 and bool((state[0] <= (tol * tol) * rr0).item()):
-            break
+        if it == check_at:
+            rr, rr_0 = torch.cat((state[0:1], rr0)).cpu().numpy()  # one read
+            if rr <= (tol * tol) * rr_0:
+                break
+            check_at = it + check_every
+            if first and 0.0 < rr < rr_0:
+                # iterations to go at the measured rate (rr falls by rr / rr_0 every `it` steps)
+                need = int(np.ceil(it * np.log((tol * tol) * rr_0 / rr) / np.log(rr / rr_0)))
+                check_at = it + max(1, min(need, max_iter - it))
+            first = False
     return z, it
 
 
@@ -125,7 +136,8 @@ class SPOMixed(OverLstsqSolver):
         x = ops.trsv_(R, z)
         res = ops.Residual(At)
         normal = ops.NormalOp(B32)
-        anorm = float(ops.nrm2(At.reshape(-1)).item())
+        anorm_d = ops.nrm2(At.reshape(-1))  # read with the first step's norms
+        anorm = None
         log.time_presolve = clock() - tic
 
         tic = clock()
@@ -135,7 +147,10 @@ class SPOMixed(OverLstsqSolver):
         for step in range(self.max_refine + 1):
             g, rss, _ = res(x, bt)
             gn = ops.nrm2(g)
-            vals = torch.cat([gn.reshape(1), rss.reshape(1)]).cpu().numpy()  # one host read per step
+            parts = [gn.reshape(1), rss.reshape(1)] + ([anorm_d.reshape(1)] if anorm is None else [])
+            vals = torch.cat(parts).cpu().numpy()  # one host read per step
+            if anorm is None:
+                anorm = float(vals[2])
             rn = np.sqrt(vals[1])
             eta = float(vals[0] / (anorm * rn)) if rn > 0 else 0.0
             etas.append(eta)

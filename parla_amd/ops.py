@@ -447,34 +447,49 @@ def cholqr2(Y, rounds=2):
     return Q
 
 
+SKETCH_NEWTON_MAX_E = 1e-2  # sketch_rz_cholqr2: second round by Newton while ||Q1^T Q1 - I||_F is below this
+
+
 def sketch_rz_cholqr2(M, n):
     """(R, z) of a sketch M = [S A | S b] (d x (n + 1)) by CholeskyQR2 on
-    S A: R1 = chol((S A)^T S A), Q1 = S A R1^{-1}, R2 = chol(Q1^T Q1),
-    Q = Q1 R2^{-1}; R = R2 R1 (positive diagonal) and z = Q^T S b, the same
-    R and Q^T S b as a Householder QR up to the signs of R's rows, to
-    O(u kappa) for kappa(S A) << u^{-1/2}.  None when S A is too
-    ill-conditioned for it (a pivot fails or R1's diagonal spread exceeds
-    CHOLQR_MAX_COND); the caller then factors M by Householder.  Two Gram
-    GEMMs, two Cholesky factorisations and two MFMA triangular solves: on
-    sketches of a few hundred rows this is much shorter than a panel-by-panel
-    Householder QR."""
+    S A: R1 = chol((S A)^T S A), Q1 = S A R1^{-1}, R2 = chol(Q1^T Q1); then
+    R = R2 R1 (positive diagonal) and z = R2^{-T} Q1^T S b = Q^T S b for
+    Q = Q1 R2^{-1}, which is never formed.  These are a Householder QR's R
+    and Q^T S b up to the signs of R's rows, to O(u kappa) for
+    kappa(S A) << u^{-1/2}.  Q1^T Q1 = I + E with ||E|| of order
+    kappa^2 u: while ||E||_F <= SKETCH_NEWTON_MAX_E, R2 = I + X comes from
+    two fixed-point steps X = H(E - X^T X) (H: the upper triangle with the
+    diagonal halved; the error after them is O(||E||^3)), else from potrf.
+    None when S A is too ill-conditioned for it (a pivot fails or R1's
+    diagonal spread exceeds CHOLQR_MAX_COND); the caller then factors M by
+    Householder.  On sketches of a few hundred rows this is much shorter than
+    a panel-by-panel Householder QR."""
     d, ncols = M.shape
     if n < 1 or n > CHOLQR_MAX_K or d < n or ncols != n + 1:
         return None
-    SA = M[:, :n]
+    SA, Sb = M[:, :n], M[:, n:]
     info = torch.zeros(2, dtype=torch.int32, device=M.device)
     G1, Q1 = _chol_round(SA, info[0:1])
-    G2, Q = _chol_round(Q1, info[1:2])
+    QtQ = gemm(Q1.t(), Q1)
+    enorm = torch.empty(1, dtype=F64, device=M.device)
+    X = small_combine(QtQ, sigma=-1.0, upper_half=True, nrm=enorm)  # H(E), ||E||_F
     stats = torch.empty(4, dtype=F64, device=M.device)
-    check(lib().pamd_cholqr_stats_f64(n, _p(G1), G1.stride(0), _p(info), None, _p(stats), stream(M.device)),
+    check(lib().pamd_cholqr_stats_f64(n, _p(G1), G1.stride(0), _p(info), _p(enorm), _p(stats), stream(M.device)),
           "pamd_cholqr_stats_f64")
-    dd = stats.cpu().numpy()
+    dd = stats.cpu().numpy()  # one read: R1's diagonal spread, ||E||_F, the first pivot
     if dd[3] != 0 or not np.all(np.isfinite(dd)) or dd[0] <= 0 or dd[1] / dd[0] > CHOLQR_MAX_COND:
         return None
-    if int(info[1].item()) != 0:
-        return None
-    R = gemm(torch.triu(G2), torch.triu(G1))
-    z = gemm(Q.t(), M[:, n:]).reshape(n)
+    w = gemm(Q1.t(), Sb).reshape(n)
+    if dd[2] <= SKETCH_NEWTON_MAX_E:
+        X = small_combine(QtQ, gemm(X.t(), X), alpha=-1.0, sigma=-1.0, upper_half=True)  # H(E - X^T X)
+        R2 = small_combine(X, sigma=1.0)  # I + X (upper triangular)
+    else:
+        potrf_(QtQ, info[1:2])
+        if int(info[1].item()) != 0:
+            return None
+        R2 = torch.triu(QtQ)
+    R = gemm(R2, torch.triu(G1))
+    z = trsv_(R2, w, trans=True)
     return R, z
 
 

@@ -142,13 +142,17 @@ def _sketch_like(d, n, cond, seed):
     return np.column_stack([SA, Sb])
 
 
+@pytest.mark.parametrize("newton", [True, False])
 @pytest.mark.parametrize("d,n,cond", [(800, 200, 1e6), (480, 120, 1e4), (300, 257, 1e2), (64, 1, 1.0)])
-def test_sketch_rz_cholqr2_matches_householder(ops, d, n, cond):
+def test_sketch_rz_cholqr2_matches_householder(ops, monkeypatch, d, n, cond, newton):
     """R and z = Q^T S b from CholeskyQR2 against LAPACK's Householder QR of the
     same sketch, after making R's diagonal positive: both are backward stable,
-    so they agree to a small multiple of u kappa(S A)."""
+    so they agree to a small multiple of u kappa(S A).  The second round by
+    its Newton form and (threshold 0) by potrf."""
     import scipy.linalg as sla
-    _say(f"sketch R, z by CholeskyQR2 d={d} n={n} cond={cond:g}")
+    _say(f"sketch R, z by CholeskyQR2 d={d} n={n} cond={cond:g} newton={newton}")
+    if not newton:
+        monkeypatch.setattr(ops, "SKETCH_NEWTON_MAX_E", 0.0)
     M = _sketch_like(d, n, cond, d + n)
     rz = ops.sketch_rz_cholqr2(_dev(M), n)
     assert rz is not None
