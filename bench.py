@@ -193,6 +193,12 @@ def main():
             spo(Al, bl, 0.0, 1e-12, 100, i, logging=False)
         torch.cuda.synchronize()
         log("profile-ls: 12 plain FP64 SPO solves done")
+        # the chosen path on the same problem, to the metric's threshold
+        chosen = pa.SPOMixed(4, npieces=4, backend="device")
+        for i in range(12):
+            chosen(Al, bl, 0.0, 1e-10, 100, i, logging=False)
+        torch.cuda.synchronize()
+        log("profile-ls: 12 SPOMixed solves done")
         return
     if args.profile_learned:
         import parla_amd.comps.sketchers.learned as lm
