@@ -292,7 +292,7 @@ class SPOLearned:
         self.iterative_solver = dsad.PcSS2()
 
     def sketch_factor(self, At, bt, rng, probs=None):
-        """Row-sampling sketch of [A | b] and its QR; returns (R, z, kappa_est)."""
+        """Row-sampling sketch of [A | b] and its R and z = Q^T S b; returns (R, z, kappa_est)."""
         At = ops.as_device(At)
         m, n = At.shape
         dev = At.device
@@ -305,9 +305,16 @@ class SPOLearned:
         it = idx if isinstance(idx, torch.Tensor) else torch.from_numpy(idx).to(dev)
         st = scale if isinstance(scale, torch.Tensor) else torch.from_numpy(scale).to(dev)
         row_gather(bt.reshape(-1, 1), it, st, Mx[:, n:])
-        fac = ops.qr_r_(Mx, n)
-        R = torch.triu(fac.R_view).contiguous()
-        z = Mx[:n, n].clone()
+        # R and Q^T S b of the sample: CholeskyQR2 when the sample is well enough
+        # conditioned for it (the learned or row-norm probabilities keep it
+        # near kappa(A)), else Householder
+        rz = ops.sketch_rz_cholqr2(Mx, n)
+        if rz is not None:
+            R, z = rz
+        else:
+            fac = ops.qr_r_(Mx, n)
+            R = torch.triu(fac.R_view).contiguous()
+            z = Mx[:n, n].clone()
         return R, z, kappa_estimate(At, R, self.lanczos_steps)
 
     def __call__(self, A, b, delta, tol, iter_lim, rng, logging=True, probs=None):
