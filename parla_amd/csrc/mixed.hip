@@ -411,13 +411,15 @@ OzShape oz_shape(int64_t d, int64_t m, int64_t ncols) {
 constexpr int NP_THREADS = 256;
 constexpr int NP_MAX_GRID = 1024;
 
-template <typename T, int NPL>
-__global__ __launch_bounds__(NP_THREADS) void normal_pass_kernel(int64_t m, int64_t n, const T* __restrict__ B,
-                                                                 int64_t ldb, const double* __restrict__ p,
-                                                                 double* __restrict__ part) {
-  __shared__ double red[NP_THREADS / 64][64 * NPL];
+// W wavefronts per workgroup: 16 for rows of at most 256 (more rows in flight
+// per workgroup for the same number of partials), 4 for wider rows (LDS)
+template <typename T, int NPL, int W>
+__global__ __launch_bounds__(64 * W) void normal_pass_kernel(int64_t m, int64_t n, const T* __restrict__ B,
+                                                             int64_t ldb, const double* __restrict__ p,
+                                                             double* __restrict__ part) {
+  __shared__ double red[W][64 * NPL];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t gw = (int64_t)gridDim.x * (NP_THREADS / 64);
+  const int64_t gw = (int64_t)gridDim.x * W;
   double pr[NPL], acc[NPL];
 #pragma unroll
   for (int q = 0; q < NPL; ++q) {
@@ -425,7 +427,7 @@ __global__ __launch_bounds__(NP_THREADS) void normal_pass_kernel(int64_t m, int6
     pr[q] = (c < n) ? p[c] : 0.0;
     acc[q] = 0.0;
   }
-  int64_t row = (int64_t)blockIdx.x * (NP_THREADS / 64) + wave;
+  int64_t row = (int64_t)blockIdx.x * W + wave;
   T a[NPL];
   if (row < m) {
 #pragma unroll
@@ -459,10 +461,10 @@ __global__ __launch_bounds__(NP_THREADS) void normal_pass_kernel(int64_t m, int6
 #pragma unroll
   for (int q = 0; q < NPL; ++q) red[wave][lane + 64 * q] = acc[q];
   __syncthreads();
-  for (int64_t c = threadIdx.x; c < n; c += NP_THREADS) {
+  for (int64_t c = threadIdx.x; c < n; c += 64 * W) {
     double t = 0.0;
 #pragma unroll
-    for (int w = 0; w < NP_THREADS / 64; ++w) t += red[w][c];
+    for (int w = 0; w < W; ++w) t += red[w][c];
     part[(int64_t)blockIdx.x * n + c] = t;
   }
 }
@@ -873,7 +875,9 @@ extern "C" int pamd_normal_pass(int dtype, int64_t m, int64_t n, const void* B, 
   const int64_t grid = np_grid(m);
   double* part = (double*)workspace;
   const int npl = (int)pamd_cdiv(n, 64);
-#define PAMD_NP(T, NPL) normal_pass_kernel<T, NPL><<<(unsigned)grid, NP_THREADS, 0, st>>>(m, n, (const T*)B, ldb, p, part)
+#define PAMD_NP(T, NPL)                                                                                       \
+  normal_pass_kernel<T, NPL, (NPL <= 4 ? 16 : 4)><<<(unsigned)grid, 64 * (NPL <= 4 ? 16 : 4), 0, st>>>(m, n, (const T*)B, \
+                                                                                                  ldb, p, part)
 #define PAMD_NP_T(T)                 \
   if (npl <= 1) PAMD_NP(T, 1);       \
   else if (npl <= 2) PAMD_NP(T, 2);  \
