@@ -118,7 +118,8 @@ def main():
                     help="only time the sketch factorisations: blocked Householder (geqrf) and the "
                          "communication-avoiding R factor (pamd_qr_r_f64), 5 calls each on the LS sketch shapes")
     ap.add_argument("--profile-ls", action="store_true",
-                    help="only run 10 plain FP64 SPO solves on A 20000 x 200 (the workload of the LS kernel trace)")
+                    help="only run 12 plain FP64 SPO and 12 SPOMixed solves on A 20000 x 200 (the workload of the LS "
+                         "kernel traces)")
     ap.add_argument("--profile-batched", action="store_true",
                     help="only run the batched solve of the 512 uneven problems twice (the workload of the batched "
                          "kernel trace)")
@@ -336,6 +337,26 @@ def main():
 
     t_lsc = timed_steps(ls_chosen_step, args.ls_steps, 2, torch)
     eta_c = orc.backward_error_extended(Al_h, bl_h, out["LSC"][0].cpu().numpy())
+    lgc = out["LSC"][1]
+    # algorithmic work of one chosen solve: the integer Gaussian grid (BF16, written and read), the split of
+    # [A | b] into 4 BF16 pieces (A read, pieces written), the 4 piece products on BF16 MFMA, the two Gram
+    # matrices and the triangular solve of CholeskyQR2 and B = A R^-1 on FP64 MFMA (A read, B written in FP32),
+    # and per refinement step one FP64 read of A (the compensated residual) plus one FP32 read of B per CG step
+    res_reads = lgc.refinement_steps + 1
+    ls_chosen_roof = roofline_terms([
+        ("gaussian_grid_write_read", 2 * 2.0 * dl * ml, HBM_PEAK_GBS * 1e9),
+        ("split_A_into_pieces", 8.0 * ml * (nl + 1) + 4 * 2.0 * ml * (nl + 1), HBM_PEAK_GBS * 1e9),
+        ("exact_sketch_4_bf16_pieces", 4 * 2.0 * dl * ml * (nl + 1), BF16_PEAK_TFLOPS * 1e12),
+        ("cholqr2_of_sketch", 2 * 2.0 * dl * nl * nl + dl * nl * nl, FP64_PEAK_TFLOPS * 1e12),
+        ("B_eq_A_Rinv", 1.0 * ml * nl * nl, FP64_PEAK_TFLOPS * 1e12),
+        ("B_read_A_write_f32", 8.0 * ml * nl + 4.0 * ml * nl, HBM_PEAK_GBS * 1e9),
+        ("residual_passes_f64", res_reads * 8.0 * ml * nl, HBM_PEAK_GBS * 1e9),
+        ("cg_normal_passes_f32", lgc.inner_iterations * 4.0 * ml * nl, HBM_PEAK_GBS * 1e9)], t_lsc)
+    ls_chosen_roof["refinement_steps"] = int(lgc.refinement_steps)
+    ls_chosen_roof["cg_iterations"] = int(lgc.inner_iterations)
+    ls_chosen_roof["dominant_kernels"] = "normal_pass_kernel + its partial reduction and CG update per CG step " \
+                                         "(three launches), the 200 x 200 triangular solves, potrf_la; see " \
+                                         "profiles/ls_chosen_20000x200_kernel_stats.md"
 
     # ------------------------------------------------------------- mixed LS
     mixed_extra = {}
@@ -636,7 +657,7 @@ def main():
             "ls_plain_f64_solves_per_s": round(1.0 / t_ls, 3), "ls_plain_f64_ms": round(t_ls * 1e3, 3),
             "ls_plain_f64_eta": eta, "ls_plain_f64_iters": ls_iters, "ls_plain_f64_roofline": ls_roof,
             "ls_chosen_solves_per_s": round(1.0 / t_lsc, 3), "ls_chosen_ms": round(t_lsc * 1e3, 3),
-            "ls_chosen_eta": eta_c,
+            "ls_chosen_eta": eta_c, "ls_chosen_roofline": ls_chosen_roof,
             "ls_chosen_workload": "same problem; SPOMixed: exact BF16 Gaussian sketch d=800 (4 pieces), FP64 QR, "
                                   "B = A R^-1 in FP32, FP64 refinement to eta <= 1e-10",
             "ls_chosen_speedup_vs_plain_f64": round(t_ls / t_lsc, 3),
