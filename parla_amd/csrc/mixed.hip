@@ -411,9 +411,14 @@ OzShape oz_shape(int64_t d, int64_t m, int64_t ncols) {
 constexpr int NP_THREADS = 256;
 constexpr int NP_MAX_GRID = 1024;
 
-// W wavefronts per workgroup: 16 for rows of at most 256 (more rows in flight
-// per workgroup for the same number of partials), 4 for wider rows (LDS)
-template <typename T, int NPL, int W>
+// W wavefronts per workgroup, D rows in flight per wavefront.  Rows of at
+// most 256: 16 wavefronts (more rows in flight for the same number of
+// partials); where the grid is at its cap (m >= 64 NP_MAX_GRID) each
+// wavefront walks many rows, and 4 in flight from a quarter of the
+// workgroups is faster (131072 x 200 FP32 with its reduction: 22.8 against
+// 27.3 us; at 20000 rows it is slower, 14.4 against 13.5).  Wider rows: 4
+// wavefronts (LDS, registers), one row ahead.
+template <typename T, int NPL, int W, int D>
 __global__ __launch_bounds__(64 * W) void normal_pass_kernel(int64_t m, int64_t n, const T* __restrict__ B,
                                                              int64_t ldb, const double* __restrict__ p,
                                                              double* __restrict__ part) {
@@ -428,35 +433,38 @@ __global__ __launch_bounds__(64 * W) void normal_pass_kernel(int64_t m, int64_t 
     acc[q] = 0.0;
   }
   int64_t row = (int64_t)blockIdx.x * W + wave;
-  T a[NPL];
-  if (row < m) {
+  // a[k]: row + k gw (zeros past the end; such a row is never reduced)
+  T a[D][NPL];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const int64_t r = row + k * gw;
 #pragma unroll
     for (int q = 0; q < NPL; ++q) {
       const int64_t c = lane + 64 * q;
-      a[q] = (c < n) ? B[row * ldb + c] : T(0);
+      a[k][q] = (r < m && c < n) ? B[r * ldb + c] : T(0);
     }
   }
   for (; row < m; row += gw) {
-    // the next row's loads are in flight while this row is reduced
-    const int64_t nrow = row + gw;
+    // row + D gw is loaded while this row is reduced
+    const int64_t nrow = row + D * gw;
     T an[NPL];
-    if (nrow < m) {
 #pragma unroll
-      for (int q = 0; q < NPL; ++q) {
-        const int64_t c = lane + 64 * q;
-        an[q] = (c < n) ? B[nrow * ldb + c] : T(0);
-      }
+    for (int q = 0; q < NPL; ++q) {
+      const int64_t c = lane + 64 * q;
+      an[q] = (nrow < m && c < n) ? B[nrow * ldb + c] : T(0);
     }
     double s = 0.0;
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) s += (double)a[q] * pr[q];
+    for (int q = 0; q < NPL; ++q) s += (double)a[0][q] * pr[q];
     s = wave_sum(s);
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) acc[q] += s * (double)a[q];
-    if (nrow < m) {
+    for (int q = 0; q < NPL; ++q) acc[q] += s * (double)a[0][q];
 #pragma unroll
-      for (int q = 0; q < NPL; ++q) a[q] = an[q];
-    }
+    for (int k = 0; k + 1 < D; ++k)
+#pragma unroll
+      for (int q = 0; q < NPL; ++q) a[k][q] = a[k + 1][q];
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) a[D - 1][q] = an[q];
   }
 #pragma unroll
   for (int q = 0; q < NPL; ++q) red[wave][lane + 64 * q] = acc[q];
@@ -757,6 +765,20 @@ int trsm_rec(int64_t m, int64_t n, const double* R, int64_t ldr, const double* X
 
 inline int64_t np_grid(int64_t m) { return imax64(1, imin64(NP_MAX_GRID, pamd_cdiv(m, 64))); }
 
+// the normal pass's launch shape for rows of 64 NPL values (see normal_pass_kernel)
+template <typename T, int NPL>
+void np_launch(unsigned grid, bool deep, hipStream_t st, int64_t m, int64_t n, const T* B, int64_t ldb,
+               const double* p, double* part) {
+  if constexpr (NPL <= 4) {
+    if (deep)
+      normal_pass_kernel<T, NPL, 16, 4><<<grid, 1024, 0, st>>>(m, n, B, ldb, p, part);
+    else
+      normal_pass_kernel<T, NPL, 16, 1><<<grid, 1024, 0, st>>>(m, n, B, ldb, p, part);
+  } else {
+    normal_pass_kernel<T, NPL, 4, 1><<<grid, 256, 0, st>>>(m, n, B, ldb, p, part);
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -872,12 +894,12 @@ extern "C" int pamd_normal_pass(int dtype, int64_t m, int64_t n, const void* B, 
   PAMD_ARG_CHECK(q != nullptr, 7);
   PAMD_ARG_CHECK(workspace != nullptr, 8);
   hipStream_t st = (hipStream_t)stream;
-  const int64_t grid = np_grid(m);
-  double* part = (double*)workspace;
   const int npl = (int)pamd_cdiv(n, 64);
-#define PAMD_NP(T, NPL)                                                                                       \
-  normal_pass_kernel<T, NPL, (NPL <= 4 ? 16 : 4)><<<(unsigned)grid, 64 * (NPL <= 4 ? 16 : 4), 0, st>>>(m, n, (const T*)B, \
-                                                                                                  ldb, p, part)
+  // rows of at most 256 with the grid at its cap: 4 rows in flight per wavefront, a quarter of the workgroups
+  const bool deep = npl <= 4 && np_grid(m) == NP_MAX_GRID;
+  const int64_t grid = deep ? np_grid(m) / 4 : np_grid(m);
+  double* part = (double*)workspace;
+#define PAMD_NP(T, NPL) np_launch<T, NPL>((unsigned)grid, deep, st, m, n, (const T*)B, ldb, p, part)
 #define PAMD_NP_T(T)                 \
   if (npl <= 1) PAMD_NP(T, 1);       \
   else if (npl <= 2) PAMD_NP(T, 2);  \

@@ -82,7 +82,8 @@ def test_ozaki_sketch_exact(ops, d, m, n, P):
 def test_normal_pass(ops, dtype):
     _say(f"normal pass {dtype}")
     rng = np.random.default_rng(3)
-    for m, n in [(5000, 200), (777, 1024), (64, 3)]:
+    # 131072 x 200: the grid at its cap (four rows in flight per wavefront)
+    for m, n in [(5000, 200), (777, 1024), (64, 3), (131072, 200), (70001, 40)]:
         B = rng.standard_normal((m, n))
         p = rng.standard_normal(n)
         Bt = _dev(B).to(dtype)
