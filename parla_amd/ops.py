@@ -408,8 +408,11 @@ def _newton_round(Q):
     m x k x k GEMM instead of potrf and trtri.  Returns (Q (I + X)^{-1}, ||E||_F)
     with ||E||_F as a one-element device tensor."""
     QtQ = gemm(Q.t(), Q)
-    enorm = torch.empty(1, dtype=F64, device=Q.device)
-    X = small_combine(QtQ, sigma=-1.0, upper_half=True, nrm=enorm)             # H(E)
+    # E, ||E||_F and H(E) by the many-workgroup combine and nrm2: the one-workgroup combine that also forms the norm
+    # moves k^2 doubles three times through one CU (the same values, 8 us slower at k = 266, 56 us at k = 512)
+    E = small_combine(QtQ, sigma=-1.0)
+    enorm = nrm2(E.reshape(-1))
+    X = small_combine(E, upper_half=True)                                       # H(E)
     X = small_combine(QtQ, gemm(X.t(), X), alpha=-1.0, sigma=-1.0, upper_half=True)  # H(E - X^T X)
     M = small_combine(gemm(X, X), X, alpha=-1.0, sigma=1.0)                     # I - X + X^2
     return gemm(Q, M), enorm
@@ -471,8 +474,9 @@ def sketch_rz_cholqr2(M, n):
     info = torch.zeros(2, dtype=torch.int32, device=M.device)
     G1, Q1 = _chol_round(SA, info[0:1])
     QtQ = gemm(Q1.t(), Q1)
-    enorm = torch.empty(1, dtype=F64, device=M.device)
-    X = small_combine(QtQ, sigma=-1.0, upper_half=True, nrm=enorm)  # H(E), ||E||_F
+    E = small_combine(QtQ, sigma=-1.0)  # as in _newton_round: E, ||E||_F, H(E)
+    enorm = nrm2(E.reshape(-1))
+    X = small_combine(E, upper_half=True)
     stats = torch.empty(4, dtype=F64, device=M.device)
     check(lib().pamd_cholqr_stats_f64(n, _p(G1), G1.stride(0), _p(info), _p(enorm), _p(stats), stream(M.device)),
           "pamd_cholqr_stats_f64")
